@@ -1,0 +1,111 @@
+/*
+ * vkhel extensions for MI355X: error control, device-side access, batched
+ * transforms and a serving engine.  The reference has no counterpart for
+ * any of these (its API is vkhel.h only); they are additions layered on the
+ * same handles, exported under the same vkhel_* version script.
+ *
+ * Functions returning int return 0 (VKHEL_OK) on success and a
+ * vkhel_status code otherwise; vkhel_ext_last_error_message() describes
+ * the last failure on the calling thread.
+ */
+#ifndef VKHEL_EXT_H
+#define VKHEL_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vkhel.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum vkhel_status {
+	VKHEL_OK = 0,
+	VKHEL_ERR_INVALID = 1,   /* bad argument (sizes, moduli, handles) */
+	VKHEL_ERR_DEVICE = 2,    /* a HIP runtime call failed */
+	VKHEL_ERR_NOMEM = 3,     /* device or host allocation failed */
+	VKHEL_ERR_BUDGET = 4,    /* serving: request exceeds the HBM budget */
+	VKHEL_ERR_CANCELLED = 5, /* serving: request cancelled */
+	VKHEL_ERR_TIMEOUT = 6,   /* serving: deadline passed before admission */
+	VKHEL_ERR_BUSY = 7,      /* serving: queue full */
+};
+
+/* ---- error control ---- */
+/* Default 1: print the message and abort(), as the reference's asserts do.
+ * With 0, failing calls return early (NULL for constructors) and record
+ * the error for the calling thread. */
+void vkhel_ext_set_abort_on_error(int enabled);
+int vkhel_ext_last_error(void);
+const char *vkhel_ext_last_error_message(void);
+void vkhel_ext_clear_error(void);
+
+/* ---- device access ---- */
+/* The hipStream_t all legacy calls of this context run on. */
+void *vkhel_ext_ctx_stream(struct vkhel_ctx *);
+int vkhel_ext_ctx_synchronize(struct vkhel_ctx *);
+/* Device name, e.g. "gfx950"; valid while the context lives. */
+const char *vkhel_ext_ctx_device_name(struct vkhel_ctx *);
+uint64_t *vkhel_ext_vector_device_ptr(struct vkhel_vector *);
+uint64_t vkhel_ext_vector_length(const struct vkhel_vector *);
+/* Synchronous device<->host copies of `count` words starting at `offset`. */
+int vkhel_ext_vector_read(const struct vkhel_vector *, uint64_t offset,
+		uint64_t count, uint64_t *host);
+int vkhel_ext_vector_write(struct vkhel_vector *, uint64_t offset,
+		uint64_t count, const uint64_t *host);
+
+/* ---- tables ---- */
+/* Reads back the reference-layout host tables (n words each). */
+int vkhel_ext_tables_info(const struct vkhel_ntt_tables *, uint64_t *n,
+		uint64_t *q, uint64_t *w);
+int vkhel_ext_tables_copy(const struct vkhel_ntt_tables *, uint64_t *roots,
+		uint64_t *inv_roots, uint64_t *roots_bf, uint64_t *inv_roots_bf);
+/* Uploads the tables to `ctx` now instead of at first use. */
+int vkhel_ext_tables_upload(struct vkhel_ctx *, struct vkhel_ntt_tables *);
+
+/* ---- NTT kernel selection ---- */
+enum vkhel_ntt_algo {
+	VKHEL_NTT_AUTO = 0,     /* autotuned choice per n (default) */
+	VKHEL_NTT_RADIX2 = 1,   /* one launch per stage (validation path) */
+	VKHEL_NTT_FUSED = 2,    /* stage-fused LDS/register kernels */
+};
+int vkhel_ext_set_ntt_algo(struct vkhel_ctx *, int algo);
+
+/* ---- batched transforms ----
+ * A batch is a list of jobs executed by one launch sequence on the context
+ * stream (captured in a hipGraph and replayed when the job list is
+ * unchanged).  A job works on an RNS polynomial: `nlimbs` limbs of n words
+ * stored back to back in one vector (limb i at word offset i*n), limb i
+ * reduced modulo tables[i]->q.  Every tables[i] of one job has the same n.
+ * Vectors must hold at least nlimbs*n words.  Inputs of NTT jobs follow the
+ * single-call rules: forward accepts any 64-bit words, inverse and polymul
+ * operands of the inverse step expect words < q.
+ */
+enum vkhel_job_op {
+	VKHEL_JOB_NTT_FWD = 0,  /* res = NTT(a)                               */
+	VKHEL_JOB_NTT_INV = 1,  /* res = INTT(a)                              */
+	VKHEL_JOB_POLYMUL = 2,  /* res = INTT(NTT(a) . NTT(b)) = a*b mod X^n+1 */
+	VKHEL_JOB_MUL = 3,      /* res = a.b mod q   (elemmul semantics)      */
+	VKHEL_JOB_ADD = 4,      /* res = (a + b) mod q for a, b < q           */
+};
+
+struct vkhel_batch;
+struct vkhel_batch *vkhel_batch_create(struct vkhel_ctx *);
+void vkhel_batch_destroy(struct vkhel_batch *);
+void vkhel_batch_clear(struct vkhel_batch *);
+/* b is ignored (may be NULL) for NTT_FWD / NTT_INV. */
+int vkhel_batch_add(struct vkhel_batch *, int op,
+		const struct vkhel_vector *a, const struct vkhel_vector *b,
+		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,
+		uint32_t nlimbs);
+/* Enqueues the batch on the context stream and returns immediately. */
+int vkhel_batch_launch(struct vkhel_batch *);
+/* vkhel_batch_launch + wait. */
+int vkhel_batch_run(struct vkhel_batch *);
+uint64_t vkhel_batch_num_jobs(const struct vkhel_batch *);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,122 @@
+"""CPU checks of the drop-in boundary: libvkhel.so loads, exports exactly the
+vkhel_* functions its headers declare, and its host-only logic (table
+construction, argument validation) matches the reference.  No call here
+touches a GPU.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vkhel_amd import lib as L
+
+REF_API = [
+    "vkhel_ctx_create", "vkhel_ctx_destroy", "vkhel_ntt_tables_create",
+    "vkhel_ntt_tables_destroy", "vkhel_vector_create", "vkhel_vector_create2",
+    "vkhel_vector_destroy", "vkhel_vector_dup", "vkhel_vector_copy_from_host",
+    "vkhel_vector_map", "vkhel_vector_unmap", "vkhel_vector_elemfma",
+    "vkhel_vector_elemmod", "vkhel_vector_elemmul", "vkhel_vector_elemgtadd",
+    "vkhel_vector_elemgtsub", "vkhel_vector_forward_transform",
+    "vkhel_vector_inverse_transform",
+]
+
+
+def _exports():
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    return sorted({line.split()[-1] for line in out.splitlines() if line.strip()})
+
+
+def test_library_loads():
+    lib = L.load()
+    assert lib is not None
+
+
+def test_reference_api_declared_and_exported():
+    declared = set(L.header_symbols())
+    exported = set(_exports())
+    for name in REF_API:
+        assert name in declared, name
+        assert name in exported, name
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    declared = set(L.header_symbols())
+    exported = set(_exports())
+    assert declared <= exported, declared - exported
+    assert declared == set(L.SIGNATURES), declared ^ set(L.SIGNATURES)
+
+
+def test_version_script_hides_internals():
+    # vkhel.syms: global vkhel_*; local *
+    others = [s for s in _exports() if not s.startswith("vkhel_")]
+    assert others == [], others
+
+
+def test_public_header_compiles_as_c_and_cpp(tmp_path):
+    inc = os.path.join(L.REPO, "include")
+    src = tmp_path / "t.c"
+    src.write_text('#include <vkhel/vkhel.h>\n#include <vkhel/vkhel_ext.h>\n'
+                   'int main(void) { return vkhel_ext_last_error(); }\n')
+    for cc in (["gcc", "-std=c11"], ["g++", "-x", "c++"]):
+        subprocess.run(cc + ["-I", inc, "-fsyntax-only", str(src)], check=True)
+
+
+# ---- host-only product logic: table builder and validation ----
+
+def _tables(n, q, w):
+    from vkhel_amd import NttTables
+    return NttTables(n, q, w)
+
+
+def test_tables_match_reference_fixtures(golden):
+    for t in golden["tables"]:
+        tab = _tables(t["n"], t["q"], t["w"])
+        r, ir, rb, irb = tab.arrays()
+        assert r.tolist() == t["roots"]
+        assert ir.tolist() == t["inv_roots"]
+        assert rb.tolist() == t["roots_bf"]
+        assert irb.tolist() == t["inv_roots_bf"]
+        tab.close()
+
+
+@pytest.mark.parametrize("n,bound", [(1 << 10, 1 << 60), (1 << 16, 1 << 50),
+                                     (1 << 17, 1 << 60)])
+def test_tables_match_oracle_at_config_sizes(n, bound):
+    q = O.primes_ntt(bound, 2 * n, 1)[0]
+    w = O.root_2n(q, n)
+    tab = _tables(n, q, w)
+    got = tab.arrays()
+    want = O.tables(n, q, w)
+    for g, e in zip(got, want):
+        assert np.array_equal(g, e)
+    tab.close()
+
+
+@pytest.mark.parametrize("n,q,w", [
+    (3, 113, 18),                 # n not a power of two
+    (1, 113, 18),                 # n < 2
+    (4, 112, 18),                 # even q
+    (4, (1 << 61) + 1, 3),        # q beyond the Barrett range
+    (4, 113, 113),                # w >= q
+    (4, 7 * 13, 13),              # w not invertible mod q
+    (128, 113, 18),               # n >= q
+])
+def test_invalid_tables_rejected(n, q, w):
+    from vkhel_amd import NttTables, VkhelError
+    with pytest.raises(VkhelError) as e:
+        NttTables(n, q, w)
+    assert e.value.code == 1
+
+
+def test_abort_mode_default_matches_reference_asserts(tmp_path):
+    # With the default error mode an invalid call aborts the process, as the
+    # reference's assert() does.
+    code = ("import ctypes; L = ctypes.CDLL(%r); "
+            "L.vkhel_ntt_tables_create.restype = ctypes.c_void_p; "
+            "L.vkhel_ntt_tables_create(3, 113, 18)" % L.LIB_PATH)
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "power of two" in r.stderr

@@ -1,0 +1,312 @@
+"""GPU parity of libvkhel.so against the reference's known answers and the
+CPU oracle.  Every call goes through the C ABI (vkhel_amd mirrors it with
+ctypes); integer results must be bit-exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import mathref as R
+import vkhel_amd as V
+
+pytestmark = pytest.mark.gpu
+
+U64MAX = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = V.Context()
+    assert c.device_name.startswith("gfx950"), c.device_name
+    yield c
+    c.close()
+
+
+def vec(ctx, data):
+    return ctx.from_array(np.asarray(data, dtype=np.uint64))
+
+
+# ---------------- test/vector.c known answers ----------------
+
+def test_ref_copy_from_host_and_dup(ctx, golden):
+    v = golden["vectors"]["vector"]
+    a = vec(ctx, v["copy_from_host"]["data"])
+    assert a.map_read().tolist() == v["copy_from_host"]["data"]
+    d = vec(ctx, v["dup"]["data"])
+    e = d.dup()
+    assert e.map_read().tolist() == v["dup"]["data"]
+    for x in (a, d, e):
+        x.close()
+
+
+def test_ref_create_zeroes(ctx):
+    z = ctx.vector(1000, zero=True)
+    assert not z.map_read().any()
+    z.close()
+
+
+def test_ref_elementwise(ctx, golden):
+    v = golden["vectors"]["vector"]
+    f = v["elemfma"]
+    a, b = vec(ctx, f["a"]), vec(ctx, f["b"])
+    for c in f["cases"]:
+        out = ctx.vector(len(f["a"]))
+        a.elemfma(b, out, c["multiplier"], f["mod"])
+        assert out.map_read().tolist() == c["expected"]
+    for c in v["elemmul"]["cases"]:
+        a, b = vec(ctx, c["a"]), vec(ctx, c["b"])
+        out = ctx.vector(len(c["a"]))
+        a.elemmul(b, out, c["mod"])
+        assert out.map_read().tolist() == c["expected"]
+    g = v["elemgtadd"]
+    a = vec(ctx, g["a"])
+    for c in g["cases"]:
+        out = ctx.vector(len(g["a"]))
+        a.elemgtadd(out, c["bound"], c["diff"])
+        assert out.map_read().tolist() == c["expected"]
+    g = v["elemgtsub"]
+    a = vec(ctx, g["a"])
+    for c in g["cases"]:
+        out = ctx.vector(len(g["a"]))
+        a.elemgtsub(out, c["bound"], c["diff"], c["mod"])
+        assert out.map_read().tolist() == c["expected"]
+    for c in v["elemmod"]["cases"]:
+        a = vec(ctx, c["a"])
+        out = ctx.vector(len(c["a"]))
+        a.elemmod(out, c["mod"], c["q"])
+        assert out.map_read().tolist() == c["expected"]
+
+
+@pytest.mark.parametrize("algo", [V.NTT_FUSED, V.NTT_RADIX2])
+def test_ref_transforms(ctx, golden, algo):
+    ctx.set_ntt_algo(algo)
+    try:
+        for c in golden["vectors"]["vector"]["transform"]["cases"]:
+            t = V.NttTables(c["n"], c["q"], c["w"])
+            if c["in_place"]:
+                x = vec(ctx, c["coeffs"])
+                x.forward_transform(x, t)
+                assert x.map_read().tolist() == c["ntt"]
+                x.inverse_transform(x, t)
+                assert x.map_read().tolist() == c["coeffs"]
+            else:
+                a = vec(ctx, c["coeffs"])
+                b = ctx.vector(c["n"])
+                a.forward_transform(b, t)
+                assert b.map_read().tolist() == c["ntt"]
+                d = ctx.vector(c["n"])
+                b.inverse_transform(d, t)
+                assert d.map_read().tolist() == c["coeffs"]
+            t.close()
+    finally:
+        ctx.set_ntt_algo(V.NTT_AUTO)
+
+
+# ---------------- elementwise vs oracle ----------------
+
+MODS = [3, 5, 769, 65537, 1125891450734593, 1152921504606830593,
+        (1 << 61) - 1, (1 << 63) + 29]
+
+
+@pytest.mark.parametrize("length", [1, 2, 7, 64, 1000, 100003])
+def test_elementwise_vs_oracle(ctx, length):
+    rng = np.random.default_rng(length)
+    raw_a = rng.integers(0, U64MAX, length, dtype=np.uint64, endpoint=True)
+    raw_b = rng.integers(0, U64MAX, length, dtype=np.uint64, endpoint=True)
+    a, b = vec(ctx, raw_a), vec(ctx, raw_b)
+    out = ctx.vector(length)
+    for q in MODS:
+        a.elemmul(b, out, q)
+        assert np.array_equal(out.read(), O.elemmul(raw_a, raw_b, q)), q
+        ca = raw_a % np.uint64(q)
+        cb = raw_b % np.uint64(q)
+        ga, gb = vec(ctx, ca), vec(ctx, cb)
+        for m in [1, 2, int(rng.integers(0, q, dtype=np.uint64))]:
+            ga.elemfma(gb, out, m, q)
+            assert np.array_equal(out.read(), O.elemfma(ca, cb, m, q)), (q, m)
+        # raw operands exercise the wrap branches of elemfma.comp
+        a.elemfma(b, out, 7, q)
+        assert np.array_equal(out.read(), O.elemfma(raw_a, raw_b, 7, q)), q
+        bound, diff = (int(x) for x in rng.integers(0, U64MAX, 2, dtype=np.uint64, endpoint=True))
+        a.elemgtsub(out, bound, diff, q)
+        assert np.array_equal(out.read(), O.elemgtsub(raw_a, bound, diff, q)), q
+        a.elemgtadd(out, bound, diff)
+        assert np.array_equal(out.read(), O.elemgtadd(raw_a, bound, diff))
+        for p in [2, 3, 17, q]:
+            a.elemmod(out, p, q)
+            assert np.array_equal(out.read(), O.elemmod(raw_a, p, q)), (p, q)
+        ga.close()
+        gb.close()
+    for x in (a, b, out):
+        x.close()
+
+
+def test_elementwise_in_place_and_length_rules(ctx):
+    rng = np.random.default_rng(3)
+    q = 1125891450734593
+    x = rng.integers(0, q, 999, dtype=np.uint64)
+    a = vec(ctx, x)
+    a.elemmul(a, a, q)
+    assert np.array_equal(a.read(), O.elemmul(x, x, q))
+    short = ctx.vector(10)
+    with pytest.raises(V.VkhelError):
+        short.elemmul(a, a, q)  # input shorter than the result
+
+
+# ---------------- transforms vs oracle ----------------
+
+def _prime(logn, bits=60):
+    return O.primes_ntt(1 << bits, 2 << logn, 1)[0]
+
+
+@pytest.mark.parametrize("algo", [V.NTT_FUSED, V.NTT_RADIX2])
+@pytest.mark.parametrize("logn", list(range(1, 18)))
+def test_ntt_vs_oracle(ctx, logn, algo):
+    n = 1 << logn
+    ctx.set_ntt_algo(algo)
+    try:
+        for bits in (50, 60):
+            q = _prime(logn, bits)
+            w = O.root_2n(q, n)
+            t = V.NttTables(n, q, w)
+            x = O.fill_uniform(n, q, seed=logn * 7 + bits)
+            a = vec(ctx, x)
+            f = ctx.vector(n)
+            a.forward_transform(f, t)
+            want = O.ntt_forward(x, q, w)
+            assert np.array_equal(f.read(), want), (logn, bits)
+            g = ctx.vector(n)
+            f.inverse_transform(g, t)
+            assert np.array_equal(g.read(), x), (logn, bits)
+            assert np.array_equal(O.ntt_inverse(want, q, w), x)
+            for v in (a, f, g):
+                v.close()
+            t.close()
+    finally:
+        ctx.set_ntt_algo(V.NTT_AUTO)
+
+
+@pytest.mark.parametrize("logn", [1, 4, 10, 13, 14, 16])
+def test_ntt_noncanonical_inputs_match_reference(ctx, logn):
+    # nttfwdbutterfly.comp reduces arbitrary words in its first stage,
+    # including sums that wrap past 2^64.
+    n = 1 << logn
+    q = _prime(logn)
+    w = O.root_2n(q, n)
+    t = V.NttTables(n, q, w)
+    rng = np.random.default_rng(logn)
+    x = rng.integers(0, U64MAX, n, dtype=np.uint64, endpoint=True)
+    x[: n // 4] = np.uint64(U64MAX) - rng.integers(0, q, n // 4, dtype=np.uint64)
+    a = vec(ctx, x)
+    a.forward_transform(a, t)
+    assert np.array_equal(a.read(), O.ntt_forward(x, q, w))
+    t.close()
+    a.close()
+
+
+@pytest.mark.parametrize("logn", [2, 3, 5, 7])
+def test_ntt_closed_form(ctx, logn):
+    n = 1 << logn
+    q = 1125899903827969
+    psi = O.root_2n(q, n)
+    t = V.NttTables(n, q, psi)
+    x = O.fill_uniform(n, q, seed=logn)
+    a = vec(ctx, x)
+    a.forward_transform(a, t)
+    assert a.read().tolist() == R.dft_negacyclic_bitrev(x.tolist(), q, psi)
+    t.close()
+
+
+def test_ntt_length_checks(ctx):
+    t = V.NttTables(16, 1125899903827969, O.root_2n(1125899903827969, 16))
+    a = ctx.vector(8)
+    with pytest.raises(V.VkhelError):
+        a.forward_transform(a, t)
+    t.close()
+
+
+# ---------------- batch API ----------------
+
+def test_batch_mixed_jobs_vs_oracle(ctx):
+    rng = np.random.default_rng(5)
+    jobs = []
+    batch = V.Batch(ctx)
+    keep = []
+    for k in range(24):
+        logn = int(rng.integers(4, 15))
+        n = 1 << logn
+        L = int(rng.integers(1, 4))
+        qs = O.primes_ntt(1 << 60, 2 * n, L)
+        ws = [O.root_2n(q, n) for q in qs]
+        tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+        op = int(rng.integers(0, 5))
+        xa = np.concatenate([O.fill_uniform(n, q, 100 * k + i) for i, q in enumerate(qs)])
+        xb = np.concatenate([O.fill_uniform(n, q, 100 * k + 50 + i) for i, q in enumerate(qs)])
+        a, b, r = vec(ctx, xa), vec(ctx, xb), ctx.vector(L * n)
+        batch.add(op, a, b, r, tabs)
+        keep += [a, b, r] + tabs
+        jobs.append((op, n, qs, ws, xa, xb, r))
+    batch.run()
+    for op, n, qs, ws, xa, xb, r in jobs:
+        got = r.read()
+        for i, (q, w) in enumerate(zip(qs, ws)):
+            sa, sb = xa[i * n:(i + 1) * n], xb[i * n:(i + 1) * n]
+            if op == V.JOB_NTT_FWD:
+                want = O.ntt_forward(sa, q, w)
+            elif op == V.JOB_NTT_INV:
+                want = O.ntt_inverse(sa, q, w)
+            elif op == V.JOB_POLYMUL:
+                want = O.polymul_ntt(sa, sb, q, w)
+            elif op == V.JOB_MUL:
+                want = O.elemmul(sa, sb, q)
+            else:
+                want = O.addmul_mod(sa, sb, 1, q)
+            assert np.array_equal(got[i * n:(i + 1) * n], want), (op, n, q)
+    # a replay of the captured graph gives the same results (no job's
+    # result vector is one of its inputs)
+    snap = [r.read() for *_, r in jobs]
+    batch.run()
+    for (*_, r), s in zip(jobs, snap):
+        assert np.array_equal(r.read(), s)
+    batch.close()
+    for x in keep:
+        x.close()
+
+
+def test_polymul_matches_schoolbook(ctx):
+    for logn in (4, 8, 11):
+        n = 1 << logn
+        q = O.primes_ntt(1 << 55, 2 * n, 1)[0]
+        w = O.root_2n(q, n)
+        t = V.NttTables(n, q, w)
+        xa, xb = O.fill_uniform(n, q, 1), O.fill_uniform(n, q, 2)
+        a, b, r = vec(ctx, xa), vec(ctx, xb), ctx.vector(n)
+        batch = V.Batch(ctx)
+        batch.add(V.JOB_POLYMUL, a, b, r, [t])
+        batch.run()
+        assert np.array_equal(r.read(), O.negacyclic_schoolbook(xa, xb, q))
+        batch.close()
+        t.close()
+
+
+def test_batch_roundtrip_full_size(ctx):
+    # 2^16 x 32 polynomials, 4 primes: NTT then INTT is the identity.
+    n, per, L = 1 << 16, 8, 4
+    qs = O.primes_ntt(1 << 50, 2 * n, L)
+    tabs = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+    rng = np.random.default_rng(9)
+    fwd, inv = V.Batch(ctx), V.Batch(ctx)
+    data = []
+    for i in range(per * L):
+        q = qs[i % L]
+        x = rng.integers(0, q, n, dtype=np.uint64)
+        v = vec(ctx, x)
+        fwd.add(V.JOB_NTT_FWD, v, None, v, [tabs[i % L]])
+        inv.add(V.JOB_NTT_INV, v, None, v, [tabs[i % L]])
+        data.append((x, v))
+    fwd.run()
+    x0, v0 = data[0]
+    assert np.array_equal(v0.read(), O.ntt_forward(x0, qs[0], tabs[0].w))
+    inv.run()
+    for x, v in data:
+        assert np.array_equal(v.read(), x)

@@ -1,0 +1,117 @@
+// Internal types of the MI355X vkhel library.  Host code is C++17 built by
+// hipcc for gfx950; nothing here is part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "vkhel/vkhel.h"
+#include "vkhel/vkhel_ext.h"
+
+namespace vk {
+
+// Records an error for the calling thread, prints it, and aborts unless
+// vkhel_ext_set_abort_on_error(0) was called.  Returns `code`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// Returns true on success; otherwise records VKHEL_ERR_DEVICE (or NOMEM).
+bool hip_ok(hipError_t e, const char *what);
+
+// Constants one (n, q) transform needs besides its twiddle tables.  Passed
+// to kernels by value.
+struct NttConsts {
+	uint64_t q;
+	uint64_t two_q;
+	uint64_t r64;     // 2^64 mod q: correction for the wrapped X + WY sum
+	uint64_t mu;      // floor(2^(bits+62) / q), bits = bitlen(q)
+	uint64_t ninv;    // n^-1 mod q
+	uint64_t ninv_p;  // Shoup companion of ninv
+	uint64_t w1n;     // inv_roots[1] * n^-1 mod q (last inverse stage)
+	uint64_t w1n_p;
+	uint32_t bshift;  // bits - 2
+	uint32_t logn;
+};
+
+// Device copy of one vkhel_ntt_tables: twiddles stored as {w, w'} pairs in
+// the reference's bit-reversed index order (roots[m + i] of stage m).
+struct DevTables {
+	uint64_t n = 0;
+	const ulonglong2 *fwd = nullptr;
+	const ulonglong2 *inv = nullptr;
+	NttConsts c{};
+	void *alloc = nullptr;
+};
+
+}  // namespace vk
+
+// Field order of the first seven members follows the reference's private
+// struct (include/priv/ntt_tables.h:6-15) so code written against it keeps
+// working; `id` keys the per-context device cache.
+struct vkhel_ntt_tables {
+	uint64_t n;
+	uint64_t q;
+	uint64_t w;
+	uint64_t *roots_of_unity;
+	uint64_t *inv_roots_of_unity;
+	uint64_t *roots_barrett_factors;
+	uint64_t *inv_roots_barrett_factors;
+	uint64_t id;
+};
+
+struct vkhel_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::string device_name;
+	int cu_count = 0;
+	int ntt_algo = VKHEL_NTT_AUTO;
+	std::mutex mu;  // guards `tables` (batch and serve threads share it)
+	std::unordered_map<uint64_t, vk::DevTables *> tables;
+};
+
+struct vkhel_vector {
+	struct vkhel_ctx *ctx;
+	uint64_t length;
+	uint64_t *dev;
+	void *staging;  // pinned host buffer while mapped
+	size_t staging_words;
+};
+
+namespace vk {
+
+// Device tables of `t` on `ctx`, uploading them on first use.  nullptr on
+// failure (error recorded).
+const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t);
+
+// One polynomial transform for the fused kernel: which words, which
+// direction's twiddles, the per-(n, q) constants and the pass split
+// (k1 = stages in pass A; 0 for n <= 2^13).
+struct NttTask {
+	const uint64_t *in;
+	uint64_t *out;
+	const ulonglong2 *tw;
+	NttConsts c;
+	uint32_t k1;
+};
+
+// Launchers (ntt.hip).  `batch` polynomials of n words, poly p at
+// in + p*n / out + p*n, all with the same tables.  Stream-ordered.
+int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
+		uint64_t *out, uint32_t batch, bool inverse, hipStream_t s);
+NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
+		bool inverse);
+// One workgroup per device-resident task, all in one direction.
+int launch_fused_tasks(const NttTask *dev_tasks, uint32_t ntasks,
+		uint32_t max_logn, bool inverse, hipStream_t s);
+
+// Pointwise launchers (elementwise.hip).
+int launch_elemmul(const uint64_t *a, const uint64_t *b, uint64_t *out,
+		uint64_t len, uint64_t q, hipStream_t s);
+int launch_addmod(const uint64_t *a, const uint64_t *b, uint64_t *out,
+		uint64_t len, uint64_t q, hipStream_t s);
+
+}  // namespace vk

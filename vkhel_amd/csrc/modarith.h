@@ -1,0 +1,56 @@
+// Device-side 64-bit modular arithmetic for gfx950.  gfx950 has no 64x64
+// multiplier: every 64-bit product below lowers to v_mad_u64_u32 /
+// v_mul_hi_u32 / v_mul_lo_u32 chains, all full rate on this chip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vk {
+
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+	return __umul64hi(a, b);
+}
+
+// x - m if x >= m else x.
+__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) {
+	return x >= m ? x - m : x;
+}
+
+// (a - b) mod q for a, b in [0, q).
+__device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b,
+		uint64_t q) {
+	return a >= b ? a - b : a + q - b;
+}
+
+// Shoup product: y * w - floor(y * wp / 2^64) * q lies in [0, 2q) for any
+// 64-bit y when w < q and wp = floor(w * 2^64 / q) (the reference's
+// nttfwdbutterfly.comp:44-48 arithmetic before its final subtraction).
+__device__ __forceinline__ uint64_t shoup_lazy(uint64_t y, uint64_t w,
+		uint64_t wp, uint64_t q) {
+	return y * w - mulhi64(y, wp) * q;
+}
+
+__device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w,
+		uint64_t wp, uint64_t q) {
+	return csub(shoup_lazy(y, w, wp, q), q);
+}
+
+// Reference 64-bit Barrett (elemmul.comp:36-47, elemgtsub.comp:36-47):
+// mu = floor(2^(bits+62) / q), shift = bits - 2, one subtraction.  Exact
+// (canonical) for every 64-bit x once bits >= 3.
+__device__ __forceinline__ uint64_t barrett64(uint64_t x, uint64_t q,
+		uint64_t mu, uint32_t shift) {
+	return csub(x - mulhi64(x >> shift, mu) * q, q);
+}
+
+// Reference 128-bit Barrett (elemmul.comp:49-60, src/numbers.c:5-28).
+// (64 - shift) & 63 reproduces x86 shift-count masking at bits == 2.
+__device__ __forceinline__ uint64_t barrett128(uint64_t hi, uint64_t lo,
+		uint64_t q, uint64_t mu, uint32_t shift) {
+	const uint64_t c = (hi << ((64 - shift) & 63)) + (lo >> shift);
+	return csub(lo - mulhi64(c, mu) * q, q);
+}
+
+}  // namespace vk

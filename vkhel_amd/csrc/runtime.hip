@@ -1,0 +1,443 @@
+// HIP runtime layer: replaces the reference's Vulkan context, VMA allocator
+// and staging transfers (src/vulkan.c, src/vector.c:1-296) and its host
+// table builder (src/ntt_tables.c).
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <set>
+#include <vector>
+
+#include "internal.h"
+#include "numbers.h"
+
+namespace vk {
+
+static int g_abort_on_error = 1;
+static thread_local int t_err = VKHEL_OK;
+static thread_local char t_msg[512];
+
+int fail(int code, const char *fmt, ...) {
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(t_msg, sizeof t_msg, fmt, ap);
+	va_end(ap);
+	t_err = code;
+	if (__atomic_load_n(&g_abort_on_error, __ATOMIC_RELAXED)) {
+		fprintf(stderr, "vkhel: error %d: %s\n", code, t_msg);
+		abort();
+	}
+	return code;
+}
+
+void clear_error() {
+	t_err = VKHEL_OK;
+	t_msg[0] = 0;
+}
+
+bool hip_ok(hipError_t e, const char *what) {
+	if (e == hipSuccess)
+		return true;
+	fail(e == hipErrorOutOfMemory ? VKHEL_ERR_NOMEM : VKHEL_ERR_DEVICE,
+			"%s: %s", what, hipGetErrorString(e));
+	return false;
+}
+
+// Live contexts, so destroying a tables object can drop its device copies.
+static std::mutex g_ctx_mu;
+static std::set<vkhel_ctx *> g_ctxs;
+static uint64_t g_next_tables_id = 1;
+
+static void free_dev_tables(DevTables *d) {
+	if (d) {
+		(void) hipFree(d->alloc);
+		delete d;
+	}
+}
+
+const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
+	if (!t)
+		return fail(VKHEL_ERR_INVALID, "null tables"), nullptr;
+	std::lock_guard<std::mutex> g(ctx->mu);
+	auto it = ctx->tables.find(t->id);
+	if (it != ctx->tables.end())
+		return it->second;
+
+	const uint64_t n = t->n, q = t->q;
+	// {w, w'} pairs for both directions.
+	std::vector<ulonglong2> host(2 * n);
+	for (uint64_t i = 0; i < n; i++) {
+		host[i] = make_ulonglong2(t->roots_of_unity[i],
+				t->roots_barrett_factors[i]);
+		host[n + i] = make_ulonglong2(t->inv_roots_of_unity[i],
+				t->inv_roots_barrett_factors[i]);
+	}
+	DevTables *d = new DevTables;
+	d->n = n;
+	if (!hip_ok(hipMalloc(&d->alloc, 2 * n * sizeof(ulonglong2)),
+				"hipMalloc(tables)")) {
+		delete d;
+		return nullptr;
+	}
+	if (!hip_ok(hipMemcpy(d->alloc, host.data(), 2 * n * sizeof(ulonglong2),
+					hipMemcpyHostToDevice), "upload tables")) {
+		free_dev_tables(d);
+		return nullptr;
+	}
+	d->fwd = (const ulonglong2 *) d->alloc;
+	d->inv = d->fwd + n;
+	NttConsts &c = d->c;
+	c.q = q;
+	c.two_q = 2 * q;
+	c.r64 = pow2_64_mod(q);
+	c.mu = barrett_mu(q);
+	c.bshift = (uint32_t) (bitlen(q) - 2);
+	c.logn = (uint32_t) (bitlen(n) - 1);
+	// src/vector.c:633: n^-1 by the reference's ext-Euclid.
+	c.ninv = invmod_euclid(n % q, q);
+	c.ninv_p = barrett_factor(c.ninv, q);
+	c.w1n = mulmod_exact(t->inv_roots_of_unity[n > 1 ? 1 : 0], c.ninv, q);
+	c.w1n_p = barrett_factor(c.w1n, q);
+	ctx->tables.emplace(t->id, d);
+	return d;
+}
+
+}  // namespace vk
+
+using namespace vk;
+
+extern "C" {
+
+void vkhel_ext_set_abort_on_error(int enabled) {
+	__atomic_store_n(&g_abort_on_error, enabled ? 1 : 0, __ATOMIC_RELAXED);
+}
+
+int vkhel_ext_last_error(void) {
+	return t_err;
+}
+
+const char *vkhel_ext_last_error_message(void) {
+	return t_msg;
+}
+
+void vkhel_ext_clear_error(void) {
+	clear_error();
+}
+
+// src/vkhel.c:4-8 + src/vulkan.c:153-206: device 0, one queue (here one
+// default-priority stream).
+struct vkhel_ctx *vkhel_ctx_create() {
+	int count = 0;
+	if (!hip_ok(hipGetDeviceCount(&count), "hipGetDeviceCount"))
+		return nullptr;
+	if (count < 1)
+		return fail(VKHEL_ERR_DEVICE, "no HIP device"), nullptr;
+	vkhel_ctx *ctx = new vkhel_ctx;
+	ctx->device = 0;
+	hipDeviceProp_t p;
+	if (!hip_ok(hipSetDevice(0), "hipSetDevice") ||
+			!hip_ok(hipGetDeviceProperties(&p, 0), "hipGetDeviceProperties") ||
+			!hip_ok(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking),
+				"hipStreamCreate")) {
+		delete ctx;
+		return nullptr;
+	}
+	ctx->device_name = p.gcnArchName;
+	ctx->cu_count = p.multiProcessorCount;
+	std::lock_guard<std::mutex> g(g_ctx_mu);
+	g_ctxs.insert(ctx);
+	return ctx;
+}
+
+void vkhel_ctx_destroy(struct vkhel_ctx *ctx) {
+	if (!ctx)
+		return;
+	{
+		std::lock_guard<std::mutex> g(g_ctx_mu);
+		g_ctxs.erase(ctx);
+	}
+	(void) hipStreamSynchronize(ctx->stream);
+	for (auto &kv : ctx->tables)
+		free_dev_tables(kv.second);
+	(void) hipStreamDestroy(ctx->stream);
+	delete ctx;
+}
+
+// src/ntt_tables.c:17-44, 65-80.  Validation the reference leaves to
+// asserts or undefined behaviour is done up front: n a power of two >= 2
+// (else bitrev indices overflow the arrays), 5 <= q < 2^61 odd (the
+// Barrett validity range, src/numbers.c:8), w < q and gcd(w, q) == 1 (else
+// the ext-Euclid loop divides by zero), n < q (inverse of n).
+struct vkhel_ntt_tables *vkhel_ntt_tables_create(uint64_t n, uint64_t q,
+		uint64_t w) {
+	if (n < 2 || (n & (n - 1)) || n > (1ull << 28))
+		return fail(VKHEL_ERR_INVALID,
+				"ntt tables: n=%llu must be a power of two in [2, 2^28]",
+				(unsigned long long) n), nullptr;
+	if (q < 5 || !(q & 1) || q >= (1ull << 61) || n >= q)
+		return fail(VKHEL_ERR_INVALID,
+				"ntt tables: q=%llu must be odd, n < q and 5 <= q < 2^61",
+				(unsigned long long) q), nullptr;
+	if (w >= q || gcd_u64(w, q) != 1)
+		return fail(VKHEL_ERR_INVALID,
+				"ntt tables: w=%llu must be < q and invertible mod q",
+				(unsigned long long) w), nullptr;
+	vkhel_ntt_tables *t = (vkhel_ntt_tables *) calloc(1, sizeof *t);
+	uint64_t *mem = (uint64_t *) malloc(4 * n * sizeof(uint64_t));
+	if (!t || !mem) {
+		free(t);
+		free(mem);
+		return fail(VKHEL_ERR_NOMEM, "ntt tables: host allocation"), nullptr;
+	}
+	t->n = n;
+	t->q = q;
+	t->w = w;
+	t->roots_of_unity = mem;
+	t->inv_roots_of_unity = mem + n;
+	t->roots_barrett_factors = mem + 2 * n;
+	t->inv_roots_barrett_factors = mem + 3 * n;
+	const uint64_t logn = bitlen(n) - 1;
+	// roots[bitrev(i)] = w^i by repeated Barrett products, as upstream.
+	// The inverse table uses w^-i = (w^-1)^i: both it and the reference's
+	// per-entry ext-Euclid yield the unique canonical inverse.
+	const uint64_t winv = invmod_euclid(w, q);
+	uint64_t r = 1, ri = 1;
+	for (uint64_t i = 0; i < n; i++) {
+		uint64_t idx = 0;
+		for (uint64_t b = 0, v = i; b < logn; b++, v >>= 1)
+			idx = (idx << 1) | (v & 1);
+		t->roots_of_unity[idx] = r;
+		t->inv_roots_of_unity[idx] = ri;
+		r = mulmod_barrett(r, w, q);
+		ri = mulmod_exact(ri, winv, q);
+	}
+	for (uint64_t i = 0; i < n; i++) {
+		t->roots_barrett_factors[i] = barrett_factor(t->roots_of_unity[i], q);
+		t->inv_roots_barrett_factors[i] =
+			barrett_factor(t->inv_roots_of_unity[i], q);
+	}
+	std::lock_guard<std::mutex> g(g_ctx_mu);
+	t->id = g_next_tables_id++;
+	return t;
+}
+
+// src/ntt_tables.c:82-87 (which leaks the struct; this frees it) plus the
+// device copies in every live context.
+void vkhel_ntt_tables_destroy(struct vkhel_ntt_tables *t) {
+	if (!t)
+		return;
+	{
+		std::lock_guard<std::mutex> g(g_ctx_mu);
+		for (vkhel_ctx *ctx : g_ctxs) {
+			std::lock_guard<std::mutex> g2(ctx->mu);
+			auto it = ctx->tables.find(t->id);
+			if (it != ctx->tables.end()) {
+				(void) hipStreamSynchronize(ctx->stream);
+				free_dev_tables(it->second);
+				ctx->tables.erase(it);
+			}
+		}
+	}
+	free(t->roots_of_unity);
+	free(t);
+}
+
+static vkhel_vector *vector_new(vkhel_ctx *ctx, uint64_t length, bool zero) {
+	if (!ctx)
+		return fail(VKHEL_ERR_INVALID, "vector_create: null ctx"), nullptr;
+	vkhel_vector *v = (vkhel_vector *) calloc(1, sizeof *v);
+	if (!v)
+		return fail(VKHEL_ERR_NOMEM, "vector_create: host allocation"), nullptr;
+	v->ctx = ctx;
+	v->length = length;
+	const size_t bytes = (length ? length : 1) * sizeof(uint64_t);
+	if (!hip_ok(hipMalloc(&v->dev, bytes), "hipMalloc(vector)")) {
+		free(v);
+		return nullptr;
+	}
+	if (zero && !(hip_ok(hipMemsetAsync(v->dev, 0, bytes, ctx->stream),
+					"hipMemsetAsync") &&
+				hip_ok(hipStreamSynchronize(ctx->stream), "sync"))) {
+		(void) hipFree(v->dev);
+		free(v);
+		return nullptr;
+	}
+	return v;
+}
+
+// src/vector.c:206-229
+struct vkhel_vector *vkhel_vector_create(struct vkhel_ctx *ctx,
+		uint64_t length) {
+	return vector_new(ctx, length, true);
+}
+
+struct vkhel_vector *vkhel_vector_create2(struct vkhel_ctx *ctx,
+		uint64_t length, bool zero) {
+	return vector_new(ctx, length, zero);
+}
+
+// src/vector.c:231-235
+void vkhel_vector_destroy(struct vkhel_vector *v) {
+	if (!v)
+		return;
+	(void) hipStreamSynchronize(v->ctx->stream);
+	if (v->staging)
+		(void) hipHostFree(v->staging);
+	(void) hipFree(v->dev);
+	free(v);
+}
+
+// src/vector.c:249-260: uninitialised allocation + device-to-device copy.
+struct vkhel_vector *vkhel_vector_dup(struct vkhel_vector *src) {
+	if (!src)
+		return fail(VKHEL_ERR_INVALID, "dup: null vector"), nullptr;
+	vkhel_vector *v = vector_new(src->ctx, src->length, false);
+	if (!v)
+		return nullptr;
+	if (!hip_ok(hipMemcpyAsync(v->dev, src->dev, src->length * 8,
+					hipMemcpyDeviceToDevice, src->ctx->stream), "dup copy") ||
+			!hip_ok(hipStreamSynchronize(src->ctx->stream), "sync")) {
+		vkhel_vector_destroy(v);
+		return nullptr;
+	}
+	return v;
+}
+
+// src/vector.c:262-268 maps, overwrites all length words, and unmaps
+// (writing them back); the net effect is this one host-to-device copy.
+void vkhel_vector_copy_from_host(struct vkhel_vector *v, const uint64_t *e) {
+	if (!v || (!e && v->length))
+		return (void) fail(VKHEL_ERR_INVALID, "copy_from_host: null argument");
+	hip_ok(hipMemcpyAsync(v->dev, e, v->length * 8, hipMemcpyHostToDevice,
+				v->ctx->stream), "copy_from_host") &&
+		hip_ok(hipStreamSynchronize(v->ctx->stream), "sync");
+}
+
+// src/vector.c:270-283: a host buffer holding the device contents.  The
+// staging buffer is max(size, length*8) bytes so callers that pass a short
+// size (examples/example.c:8,32,40) still get a valid mapping; the first
+// min(size, length*8) bytes are copied from the device.
+void vkhel_vector_map(struct vkhel_vector *v, void **mem, size_t size) {
+	if (!v || !mem)
+		return (void) fail(VKHEL_ERR_INVALID, "map: null argument");
+	if (v->staging)
+		return (void) fail(VKHEL_ERR_INVALID, "map: vector already mapped");
+	const size_t words = std::max<size_t>((size + 7) / 8, v->length);
+	void *p = nullptr;
+	if (!hip_ok(hipHostMalloc(&p, std::max<size_t>(words, 1) * 8, 0),
+				"hipHostMalloc(map)"))
+		return;
+	const size_t copy = std::min<size_t>(size, v->length * 8);
+	if (copy && !(hip_ok(hipMemcpyAsync(p, v->dev, copy, hipMemcpyDeviceToHost,
+						v->ctx->stream), "map copy") &&
+				hip_ok(hipStreamSynchronize(v->ctx->stream), "sync"))) {
+		(void) hipHostFree(p);
+		return;
+	}
+	v->staging = p;
+	v->staging_words = words;
+	*mem = p;
+}
+
+// src/vector.c:285-296: writes length words back, frees the staging.
+void vkhel_vector_unmap(struct vkhel_vector *v) {
+	if (!v || !v->staging)
+		return (void) fail(VKHEL_ERR_INVALID, "unmap: vector not mapped");
+	hip_ok(hipMemcpyAsync(v->dev, v->staging, v->length * 8,
+				hipMemcpyHostToDevice, v->ctx->stream), "unmap copy") &&
+		hip_ok(hipStreamSynchronize(v->ctx->stream), "sync");
+	(void) hipHostFree(v->staging);
+	v->staging = nullptr;
+	v->staging_words = 0;
+}
+
+void *vkhel_ext_ctx_stream(struct vkhel_ctx *ctx) {
+	return ctx ? (void *) ctx->stream : nullptr;
+}
+
+int vkhel_ext_ctx_synchronize(struct vkhel_ctx *ctx) {
+	if (!ctx)
+		return fail(VKHEL_ERR_INVALID, "null ctx");
+	return hip_ok(hipStreamSynchronize(ctx->stream), "sync") ? VKHEL_OK
+		: VKHEL_ERR_DEVICE;
+}
+
+const char *vkhel_ext_ctx_device_name(struct vkhel_ctx *ctx) {
+	return ctx ? ctx->device_name.c_str() : "";
+}
+
+uint64_t *vkhel_ext_vector_device_ptr(struct vkhel_vector *v) {
+	return v ? v->dev : nullptr;
+}
+
+uint64_t vkhel_ext_vector_length(const struct vkhel_vector *v) {
+	return v ? v->length : 0;
+}
+
+int vkhel_ext_vector_read(const struct vkhel_vector *v, uint64_t offset,
+		uint64_t count, uint64_t *host) {
+	if (!v || (!host && count) || offset > v->length ||
+			count > v->length - offset)
+		return fail(VKHEL_ERR_INVALID, "vector_read: range out of bounds");
+	if (count && !(hip_ok(hipMemcpyAsync(host, v->dev + offset, count * 8,
+						hipMemcpyDeviceToHost, v->ctx->stream), "read") &&
+				hip_ok(hipStreamSynchronize(v->ctx->stream), "sync")))
+		return VKHEL_ERR_DEVICE;
+	return VKHEL_OK;
+}
+
+int vkhel_ext_vector_write(struct vkhel_vector *v, uint64_t offset,
+		uint64_t count, const uint64_t *host) {
+	if (!v || (!host && count) || offset > v->length ||
+			count > v->length - offset)
+		return fail(VKHEL_ERR_INVALID, "vector_write: range out of bounds");
+	if (count && !(hip_ok(hipMemcpyAsync(v->dev + offset, host, count * 8,
+						hipMemcpyHostToDevice, v->ctx->stream), "write") &&
+				hip_ok(hipStreamSynchronize(v->ctx->stream), "sync")))
+		return VKHEL_ERR_DEVICE;
+	return VKHEL_OK;
+}
+
+int vkhel_ext_tables_info(const struct vkhel_ntt_tables *t, uint64_t *n,
+		uint64_t *q, uint64_t *w) {
+	if (!t)
+		return fail(VKHEL_ERR_INVALID, "null tables");
+	if (n)
+		*n = t->n;
+	if (q)
+		*q = t->q;
+	if (w)
+		*w = t->w;
+	return VKHEL_OK;
+}
+
+int vkhel_ext_tables_copy(const struct vkhel_ntt_tables *t, uint64_t *roots,
+		uint64_t *inv_roots, uint64_t *roots_bf, uint64_t *inv_roots_bf) {
+	if (!t)
+		return fail(VKHEL_ERR_INVALID, "null tables");
+	const size_t b = t->n * sizeof(uint64_t);
+	if (roots)
+		memcpy(roots, t->roots_of_unity, b);
+	if (inv_roots)
+		memcpy(inv_roots, t->inv_roots_of_unity, b);
+	if (roots_bf)
+		memcpy(roots_bf, t->roots_barrett_factors, b);
+	if (inv_roots_bf)
+		memcpy(inv_roots_bf, t->inv_roots_barrett_factors, b);
+	return VKHEL_OK;
+}
+
+int vkhel_ext_tables_upload(struct vkhel_ctx *ctx, struct vkhel_ntt_tables *t) {
+	if (!ctx)
+		return fail(VKHEL_ERR_INVALID, "null ctx");
+	return device_tables(ctx, t) ? VKHEL_OK : vkhel_ext_last_error();
+}
+
+int vkhel_ext_set_ntt_algo(struct vkhel_ctx *ctx, int algo) {
+	if (!ctx || algo < VKHEL_NTT_AUTO || algo > VKHEL_NTT_FUSED)
+		return fail(VKHEL_ERR_INVALID, "set_ntt_algo: bad argument");
+	ctx->ntt_algo = algo;
+	return VKHEL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+"""Python mirror of the vkhel C API (include/vkhel/vkhel.h).
+
+Names and argument meaning follow the reference's public functions
+(include/vkhel/vkhel.h:8-53): ``Context`` is ``vkhel_ctx``, ``NttTables``
+is ``vkhel_ntt_tables``, ``Vector`` is ``vkhel_vector`` and its methods are
+the ``vkhel_vector_*`` calls.  Errors raise ``VkhelError`` instead of
+asserting.
+"""
+import ctypes
+
+import numpy as np
+
+from . import lib as _lib
+
+NTT_AUTO, NTT_RADIX2, NTT_FUSED = 0, 1, 2
+JOB_NTT_FWD, JOB_NTT_INV, JOB_POLYMUL, JOB_MUL, JOB_ADD = range(5)
+
+
+def _L():
+    return _lib.load()
+
+
+def _u64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+class Context:
+    """vkhel_ctx_create / vkhel_ctx_destroy (src/vkhel.c:4-13)."""
+
+    def __init__(self):
+        L = _L()
+        self._h = L.vkhel_ctx_create()
+        if not self._h:
+            _lib.check()
+            raise _lib.VkhelError(2, "vkhel_ctx_create failed")
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def device_name(self):
+        return _L().vkhel_ext_ctx_device_name(self._h).decode()
+
+    @property
+    def stream(self):
+        return _L().vkhel_ext_ctx_stream(self._h)
+
+    def synchronize(self):
+        _lib.check(_L().vkhel_ext_ctx_synchronize(self._h))
+
+    def set_ntt_algo(self, algo):
+        _lib.check(_L().vkhel_ext_set_ntt_algo(self._h, algo))
+
+    def vector(self, length, zero=True):
+        return Vector(self, length, zero)
+
+    def from_array(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        v = Vector(self, a.size, zero=False)
+        v.copy_from_host(a)
+        return v
+
+    def close(self):
+        if self._h:
+            _L().vkhel_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class NttTables:
+    """vkhel_ntt_tables_create(n, q, w) (src/ntt_tables.c:65-80)."""
+
+    def __init__(self, n, q, w):
+        self.n, self.q, self.w = int(n), int(q), int(w)
+        self._h = _L().vkhel_ntt_tables_create(self.n, self.q, self.w)
+        if not self._h:
+            _lib.check()
+            raise _lib.VkhelError(1, "vkhel_ntt_tables_create failed")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def arrays(self):
+        """(roots, inv_roots, roots_bf, inv_roots_bf), reference layout."""
+        out = [np.zeros(self.n, dtype=np.uint64) for _ in range(4)]
+        _lib.check(_L().vkhel_ext_tables_copy(self._h, *[_u64p(a) for a in out]))
+        return tuple(out)
+
+    def close(self):
+        if self._h:
+            _L().vkhel_ntt_tables_destroy(self._h)
+            self._h = None
+
+
+class Vector:
+    """vkhel_vector and its operations (src/vector.c)."""
+
+    def __init__(self, ctx, length, zero=True, _handle=None):
+        self.ctx = ctx
+        if _handle is not None:
+            self._h = _handle
+        else:
+            self._h = _L().vkhel_vector_create2(ctx.handle, int(length), bool(zero))
+            if not self._h:
+                _lib.check()
+                raise _lib.VkhelError(3, "vkhel_vector_create2 failed")
+        self.length = int(_L().vkhel_ext_vector_length(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def copy_from_host(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        if a.size < self.length:
+            raise ValueError("host array shorter than the vector")
+        _L().vkhel_vector_copy_from_host(self._h, _u64p(a))
+        _lib.check()
+
+    def map_read(self):
+        """vkhel_vector_map + copy out + vkhel_vector_unmap."""
+        L = _L()
+        p = ctypes.c_void_p()
+        L.vkhel_vector_map(self._h, ctypes.byref(p), self.length * 8)
+        _lib.check()
+        out = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint64)),
+                                    shape=(max(self.length, 1),))[:self.length].copy()
+        L.vkhel_vector_unmap(self._h)
+        _lib.check()
+        return out
+
+    def read(self, offset=0, count=None):
+        count = self.length - offset if count is None else count
+        out = np.empty(count, dtype=np.uint64)
+        _lib.check(_L().vkhel_ext_vector_read(self._h, offset, count, _u64p(out)))
+        return out
+
+    def write(self, arr, offset=0):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        _lib.check(_L().vkhel_ext_vector_write(self._h, offset, a.size, _u64p(a)))
+
+    def dup(self):
+        h = _L().vkhel_vector_dup(self._h)
+        if not h:
+            _lib.check()
+        return Vector(self.ctx, 0, _handle=h)
+
+    def device_ptr(self):
+        return _L().vkhel_ext_vector_device_ptr(self._h)
+
+    # ---- ops: self is the first operand, `out` the result vector ----
+    def elemfma(self, b, out, multiplier, mod):
+        _L().vkhel_vector_elemfma(self._h, b.handle, out.handle, multiplier, mod)
+        _lib.check()
+
+    def elemmod(self, out, mod, q):
+        _L().vkhel_vector_elemmod(self._h, out.handle, mod, q)
+        _lib.check()
+
+    def elemmul(self, b, out, mod):
+        _L().vkhel_vector_elemmul(self._h, b.handle, out.handle, mod)
+        _lib.check()
+
+    def elemgtadd(self, out, bound, diff):
+        _L().vkhel_vector_elemgtadd(self._h, out.handle, bound, diff)
+        _lib.check()
+
+    def elemgtsub(self, out, bound, diff, mod):
+        _L().vkhel_vector_elemgtsub(self._h, out.handle, bound, diff, mod)
+        _lib.check()
+
+    def forward_transform(self, out, tables):
+        _L().vkhel_vector_forward_transform(self._h, out.handle, tables.handle)
+        _lib.check()
+
+    def inverse_transform(self, out, tables):
+        _L().vkhel_vector_inverse_transform(self._h, out.handle, tables.handle)
+        _lib.check()
+
+    def close(self):
+        if self._h:
+            _L().vkhel_vector_destroy(self._h)
+            self._h = None
+
+
+class Batch:
+    """vkhel_batch_* (include/vkhel/vkhel_ext.h): many RNS jobs, one graph."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self._h = _L().vkhel_batch_create(ctx.handle)
+        if not self._h:
+            _lib.check()
+
+    def add(self, op, a, b, res, tables):
+        tabs = list(tables)
+        arr = (ctypes.c_void_p * len(tabs))(*[t.handle for t in tabs])
+        _lib.check(_L().vkhel_batch_add(self._h, op, a.handle,
+                                        b.handle if b is not None else None,
+                                        res.handle, arr, len(tabs)))
+
+    def clear(self):
+        _L().vkhel_batch_clear(self._h)
+
+    def launch(self):
+        _lib.check(_L().vkhel_batch_launch(self._h))
+
+    def run(self):
+        _lib.check(_L().vkhel_batch_run(self._h))
+
+    def __len__(self):
+        return int(_L().vkhel_batch_num_jobs(self._h))
+
+    def close(self):
+        if self._h:
+            _L().vkhel_batch_destroy(self._h)
+            self._h = None
