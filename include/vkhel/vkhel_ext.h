@@ -71,6 +71,22 @@ enum vkhel_ntt_algo {
 };
 int vkhel_ext_set_ntt_algo(struct vkhel_ctx *, int algo);
 
+/* ---- autotuning of the fused NTT kernels ----
+ * Sizes above 2^11 run as two fused passes over tiles of 2^11, 2^12 or
+ * 2^13 words (the tile size fixes the stage split and workgroup shape).
+ * vkhel_ext_autotune times each compiled tile size for every log2 n in
+ * [logn_lo, logn_hi] (3..20) on `batch` synthetic polynomials (0: enough
+ * to fill the device) and keeps the fastest per size on this context. */
+int vkhel_ext_autotune(struct vkhel_ctx *, uint32_t logn_lo, uint32_t logn_hi,
+		uint32_t batch);
+/* Tile size (log2 words) in use for log2 n, or -1 on error. */
+int vkhel_ext_tile_bits(struct vkhel_ctx *, uint32_t logn);
+/* tile_bits 11..13, or 0 for the built-in default. */
+int vkhel_ext_set_tile_bits(struct vkhel_ctx *, uint32_t logn, int tile_bits);
+/* Tuned choices as text lines "log2n tile_bits". */
+int vkhel_ext_autotune_save(struct vkhel_ctx *, const char *path);
+int vkhel_ext_autotune_load(struct vkhel_ctx *, const char *path);
+
 /* ---- batched transforms ----
  * A batch is a list of jobs executed by one launch sequence on the context
  * stream (captured in a hipGraph and replayed when the job list is

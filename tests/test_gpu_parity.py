@@ -310,3 +310,61 @@ def test_batch_roundtrip_full_size(ctx):
     inv.run()
     for x, v in data:
         assert np.array_equal(v.read(), x)
+
+
+# ---------------- tile sizes, split vs whole-polynomial kernels ----------------
+
+@pytest.mark.parametrize("tb", [11, 12, 13])
+@pytest.mark.parametrize("logn", [12, 13, 14, 16, 17, 18])
+def test_tile_sizes_per_call_and_batched(ctx, logn, tb):
+    n = 1 << logn
+    q = _prime(logn, 55)
+    w = O.root_2n(q, n)
+    t = V.NttTables(n, q, w)
+    ctx.set_tile_bits(logn, tb)
+    try:
+        # per-call path: one polynomial, split passes (one workgroup per tile)
+        x = O.fill_uniform(n, q, seed=logn + tb)
+        a = vec(ctx, x)
+        f = ctx.vector(n)
+        a.forward_transform(f, t)
+        want = O.ntt_forward(x, q, w)
+        assert np.array_equal(f.read(), want)
+        f.inverse_transform(f, t)
+        assert np.array_equal(f.read(), x)
+        # large batch: one workgroup per polynomial, two fused passes
+        npoly = 80
+        rng = np.random.default_rng(logn * 100 + tb)
+        xs = [rng.integers(0, q, n, dtype=np.uint64) for _ in range(npoly)]
+        vs = [vec(ctx, z) for z in xs]
+        fb, ib = V.Batch(ctx), V.Batch(ctx)
+        for v in vs:
+            fb.add(V.JOB_NTT_FWD, v, None, v, [t])
+            ib.add(V.JOB_NTT_INV, v, None, v, [t])
+        fb.run()
+        for i in (0, npoly // 2, npoly - 1):
+            assert np.array_equal(vs[i].read(), O.ntt_forward(xs[i], q, w)), i
+        ib.run()
+        for z, v in zip(xs, vs):
+            assert np.array_equal(v.read(), z)
+        for v in vs + [a, f]:
+            v.close()
+        fb.close()
+        ib.close()
+    finally:
+        ctx.set_tile_bits(logn, 0)
+        t.close()
+
+
+def test_autotune_choices_and_cache(ctx, tmp_path):
+    ctx.autotune(12, 14, 64)
+    chosen = {l: ctx.tile_bits(l) for l in (12, 13, 14)}
+    assert all(tb in (11, 12, 13) for tb in chosen.values()), chosen
+    path = tmp_path / "tune.txt"
+    ctx.autotune_save(path)
+    for l in chosen:
+        ctx.set_tile_bits(l, 0)
+    ctx.autotune_load(path)
+    assert {l: ctx.tile_bits(l) for l in chosen} == chosen
+    for l in chosen:
+        ctx.set_tile_bits(l, 0)

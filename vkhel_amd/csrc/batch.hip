@@ -10,6 +10,7 @@
 //
 // The launch sequence is captured once into a hipGraph and replayed while
 // the job list is unchanged.
+#include <algorithm>
 #include <vector>
 
 #include "internal.h"
@@ -69,6 +70,13 @@ __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks)
 
 }  // namespace vk
 
+namespace vk {
+// A run of tasks with one log2 n inside a direction's task array.
+struct Segment {
+	uint32_t off, count, logn;
+};
+}  // namespace vk
+
 struct vkhel_batch {
 	vkhel_ctx *ctx;
 	std::vector<vk::Job> jobs;
@@ -76,8 +84,9 @@ struct vkhel_batch {
 	// Device state built at launch.
 	vk::NttTask *d_fwd = nullptr, *d_inv = nullptr;
 	vk::PointTask *d_pt = nullptr;
-	uint32_t n_fwd = 0, n_inv = 0, n_pt = 0;
-	uint32_t fwd_logn = 0, inv_logn = 0;
+	std::vector<vk::NttTask> h_fwd, h_inv;
+	std::vector<vk::Segment> seg_fwd, seg_inv;
+	uint32_t n_pt = 0;
 	uint64_t pt_maxlen = 0;
 	uint64_t *scratch = nullptr;
 	size_t scratch_words = 0;
@@ -111,10 +120,38 @@ static bool upload(const std::vector<T> &h, T **d) {
 				hipMemcpyHostToDevice), "upload batch tasks");
 }
 
+// Sorts tasks by log2 n and records the runs.
+static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
+	std::stable_sort(tasks.begin(), tasks.end(),
+			[](const NttTask &a, const NttTask &b) { return a.c.logn < b.c.logn; });
+	std::vector<Segment> segs;
+	for (uint32_t i = 0; i < tasks.size(); i++) {
+		if (segs.empty() || segs.back().logn != tasks[i].c.logn)
+			segs.push_back(Segment{i, 0, tasks[i].c.logn});
+		segs.back().count++;
+	}
+	return segs;
+}
+
+static int launch_segments(const vkhel_ctx *ctx,
+		const std::vector<Segment> &segs, const std::vector<NttTask> &host,
+		const NttTask *dev, bool inverse, hipStream_t s) {
+	for (const Segment &g : segs) {
+		int rc = VKHEL_OK;
+		if (ntt_fused_supported(g.logn)) {
+			rc = launch_fused_tasks(ctx, dev + g.off, g.count, g.logn, inverse, s);
+		} else {
+			for (uint32_t i = 0; i < g.count && rc == VKHEL_OK; i++)
+				rc = launch_task_ntt(host[g.off + i], inverse, s);
+		}
+		if (rc != VKHEL_OK)
+			return rc;
+	}
+	return VKHEL_OK;
+}
+
 static int enqueue(vkhel_batch *b, hipStream_t s) {
-	int rc = VKHEL_OK;
-	if (b->n_fwd)
-		rc = launch_fused_tasks(b->d_fwd, b->n_fwd, b->fwd_logn, false, s);
+	int rc = launch_segments(b->ctx, b->seg_fwd, b->h_fwd, b->d_fwd, false, s);
 	if (rc == VKHEL_OK && b->n_pt) {
 		const uint64_t blocks = std::min<uint64_t>((b->pt_maxlen / 2 + 255) / 256,
 				64);
@@ -123,8 +160,8 @@ static int enqueue(vkhel_batch *b, hipStream_t s) {
 		rc = hip_ok(hipGetLastError(), "pointwise launch") ? VKHEL_OK
 			: VKHEL_ERR_DEVICE;
 	}
-	if (rc == VKHEL_OK && b->n_inv)
-		rc = launch_fused_tasks(b->d_inv, b->n_inv, b->inv_logn, true, s);
+	if (rc == VKHEL_OK)
+		rc = launch_segments(b->ctx, b->seg_inv, b->h_inv, b->d_inv, true, s);
 	return rc;
 }
 
@@ -134,7 +171,6 @@ static int build(vkhel_batch *b) {
 	vkhel_ctx *ctx = b->ctx;
 	std::vector<NttTask> fwd, inv;
 	std::vector<PointTask> pt;
-	uint32_t fwd_logn = 0, inv_logn = 0;
 	uint64_t pt_maxlen = 0;
 	size_t scratch_words = 0;
 	for (const Job &j : b->jobs)
@@ -158,26 +194,21 @@ static int build(vkhel_batch *b) {
 				return vkhel_ext_last_error();
 			const uint64_t off = l * n;
 			const uint64_t q = t->c.q;
-			const uint32_t logn = t->c.logn;
 			switch (j.op) {
 			case VKHEL_JOB_NTT_FWD:
 				fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
-				fwd_logn = std::max(fwd_logn, logn);
 				break;
 			case VKHEL_JOB_NTT_INV:
 				inv.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, true));
-				inv_logn = std::max(inv_logn, logn);
 				break;
 			case VKHEL_JOB_POLYMUL: {
 				uint64_t *tmp = sp;
 				sp += n;
 				fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
 				fwd.push_back(make_task(*t, j.b->dev + off, tmp, false));
-				fwd_logn = std::max(fwd_logn, logn);
 				pt.push_back(PointTask{j.res->dev + off, tmp, j.res->dev + off, n, q,
 						barrett_mu(q), (uint32_t) (bitlen(q) - 2), 0});
 				inv.push_back(make_task(*t, j.res->dev + off, j.res->dev + off, true));
-				inv_logn = std::max(inv_logn, logn);
 				break;
 			}
 			case VKHEL_JOB_MUL:
@@ -191,14 +222,14 @@ static int build(vkhel_batch *b) {
 			pt_maxlen = std::max<uint64_t>(pt_maxlen, n);
 		}
 	}
+	b->seg_fwd = segment(fwd);
+	b->seg_inv = segment(inv);
 	if (!upload(fwd, &b->d_fwd) || !upload(inv, &b->d_inv) ||
 			!upload(pt, &b->d_pt))
 		return vkhel_ext_last_error();
-	b->n_fwd = (uint32_t) fwd.size();
-	b->n_inv = (uint32_t) inv.size();
+	b->h_fwd = std::move(fwd);
+	b->h_inv = std::move(inv);
 	b->n_pt = (uint32_t) pt.size();
-	b->fwd_logn = fwd_logn;
-	b->inv_logn = inv_logn;
 	b->pt_maxlen = pt_maxlen;
 
 	// Capture the launch sequence once; replays skip per-kernel launch cost.

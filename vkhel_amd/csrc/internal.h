@@ -8,6 +8,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "vkhel/vkhel.h"
 #include "vkhel/vkhel_ext.h"
@@ -69,6 +70,8 @@ struct vkhel_ctx {
 	std::string device_name;
 	int cu_count = 0;
 	int ntt_algo = VKHEL_NTT_AUTO;
+	// Tuned tile size (log2 words) per log2 n; 0 = default.
+	std::vector<int> tile_bits = std::vector<int>(64, 0);
 	std::mutex mu;  // guards `tables` (batch and serve threads share it)
 	std::unordered_map<uint64_t, vk::DevTables *> tables;
 };
@@ -95,8 +98,11 @@ struct NttTask {
 	uint64_t *out;
 	const ulonglong2 *tw;
 	NttConsts c;
-	uint32_t k1;
 };
+
+// Tile size (log2 words) of the fused two-pass kernels when no tuned value
+// exists for a size.
+constexpr int kDefaultTileBits = 13;
 
 // Launchers (ntt.hip).  `batch` polynomials of n words, poly p at
 // in + p*n / out + p*n, all with the same tables.  Stream-ordered.
@@ -104,9 +110,15 @@ int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
 		uint64_t *out, uint32_t batch, bool inverse, hipStream_t s);
 NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
 		bool inverse);
-// One workgroup per device-resident task, all in one direction.
-int launch_fused_tasks(const NttTask *dev_tasks, uint32_t ntasks,
-		uint32_t max_logn, bool inverse, hipStream_t s);
+// Device-resident tasks that share log2 n (which selects the compiled
+// kernel) and direction; the context supplies the tuned tile size.
+int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
+		uint32_t ntasks, uint32_t logn, bool inverse, hipStream_t s);
+// Tile size (log2 words) used for log2 n on this context.
+int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn);
+bool ntt_fused_supported(uint32_t logn);
+// Radix-2 path for one task (sizes the fused kernels do not cover).
+int launch_task_ntt(const NttTask &k, bool inverse, hipStream_t s);
 
 // Pointwise launchers (elementwise.hip).
 int launch_elemmul(const uint64_t *a, const uint64_t *b, uint64_t *out,

@@ -37,6 +37,28 @@ __device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w,
 	return csub(shoup_lazy(y, w, wp, q), q);
 }
 
+// Shoup product on 32-bit halves, result in [0, 2q) like shoup_lazy:
+//   Q = floor(y * wp / 2^64) exactly, R = y*w + Q*(2^64 - q) mod 2^64.
+// nq = 2^64 - q.  Written out so every partial product is one
+// v_mad_u64_u32 / v_mul_hi_u32 / v_mul_lo_u32 and the final subtraction
+// folds into multiply-adds (no borrow chain).
+__device__ __forceinline__ uint64_t shoup32(uint64_t y, uint64_t w,
+		uint64_t wp, uint64_t nq) {
+	const uint32_t y0 = (uint32_t) y, y1 = (uint32_t) (y >> 32);
+	const uint32_t p0 = (uint32_t) wp, p1 = (uint32_t) (wp >> 32);
+	const uint64_t a = __umulhi(y0, p0);
+	const uint64_t b = (uint64_t) y0 * p1 + a;             // < 2^64
+	const uint64_t m = (uint64_t) y1 * p0 + (uint32_t) b;  // < 2^64
+	const uint64_t qh = (uint64_t) y1 * p1 + (b >> 32) + (m >> 32);
+	const uint32_t w0 = (uint32_t) w, w1 = (uint32_t) (w >> 32);
+	const uint32_t n0 = (uint32_t) nq, n1 = (uint32_t) (nq >> 32);
+	const uint32_t q0 = (uint32_t) qh, q1 = (uint32_t) (qh >> 32);
+	const uint64_t r = (uint64_t) y0 * w0 + (uint64_t) q0 * n0;
+	const uint32_t rh = (uint32_t) (r >> 32) + y0 * w1 + y1 * w0 + q0 * n1 +
+		q1 * n0;
+	return ((uint64_t) rh << 32) | (uint32_t) r;
+}
+
 // Reference 64-bit Barrett (elemmul.comp:36-47, elemgtsub.comp:36-47):
 // mu = floor(2^(bits+62) / q), shift = bits - 2, one subtraction.  Exact
 // (canonical) for every 64-bit x once bits >= 3.

@@ -1,0 +1,370 @@
+// Stage-fused NTT engine for gfx950 (device templates).  Included by the
+// per-tile-size instantiation units ntt_fused_kt*.hip.
+#pragma once
+
+#include <array>
+#include <utility>
+
+#include "internal.h"
+#include "modarith.h"
+
+namespace vk {
+
+typedef void (*FusedKernel)(NttTask, const NttTask *);
+
+typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
+
+// One compiled (log2 n, tile size) variant: whole-transform kernels for both
+// directions (one workgroup per polynomial), the two passes as separate
+// kernels (one workgroup per tile, for small batches), and launch shape.
+struct FusedEntry {
+	FusedKernel fwd, inv;
+	PassKernel fwd_a, fwd_b, inv_b, inv_a;
+	int threads;
+	int tiles;  // tiles per pass (1: one-tile size, passes unused)
+};
+
+constexpr int kMaxFusedLogn = 20;
+constexpr int kMinFusedLogn = 3;
+constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
+
+
+// Lazy butterflies.  Forward (Cooley-Tukey): x, y in [0, 4q) -> [0, 4q).
+// Inverse (Gentleman-Sande): x, y in [0, 2q) -> [0, 2q).  Both stay
+// congruent mod q to the reference's canonical butterflies
+// (nttfwdbutterfly.comp:44-57, nttrevbutterfly.comp:44-57).
+__device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
+		uint64_t two_q, uint64_t nq) {
+	const uint64_t xr = csub(x, two_q);
+	const uint64_t t = shoup32(y, w.x, w.y, nq);
+	x = xr + t;
+	y = xr + two_q - t;
+}
+
+__device__ __forceinline__ void bf_inv(uint64_t &x, uint64_t &y, ulonglong2 w,
+		uint64_t two_q, uint64_t nq) {
+	const uint64_t s = csub(x + y, two_q);
+	const uint64_t d = x + two_q - y;
+	x = s;
+	y = shoup32(d, w.x, w.y, nq);
+}
+
+// First forward stage on arbitrary 64-bit words, exactly as the reference:
+// (X + WY) mod 2^64 mod q and (X - WY) mod q with WY canonical.
+__device__ __forceinline__ void bf_fwd_first(uint64_t &x, uint64_t &y,
+		ulonglong2 w, const NttConsts &c) {
+	const uint64_t X = x;
+	const uint64_t wy = shoup(y, w.x, w.y, c.q);
+	const uint64_t xr = barrett64(X, c.q, c.mu, c.bshift);
+	uint64_t ox = csub(xr + wy, c.q);
+	if (X + wy < X)
+		ox = sub_mod(ox, c.r64, c.q);
+	x = ox;
+	y = sub_mod(xr, wy, c.q);
+}
+
+// Last inverse stage (m = 1) with the reference's final multiplication by
+// n^-1 (src/vector.c:633-638) folded in; canonical outputs.
+__device__ __forceinline__ void bf_inv_last(uint64_t &x, uint64_t &y,
+		const NttConsts &c) {
+	const uint64_t s = csub(x + y, c.two_q);
+	const uint64_t d = x + c.two_q - y;
+	x = shoup(s, c.ninv, c.ninv_p, c.q);
+	y = shoup(d, c.w1n, c.w1n_p, c.q);
+}
+
+// Tiles hold 2^KT words of one polynomial, 8 per thread.  A pass runs a
+// contiguous range of butterfly stages on every tile of the polynomial:
+//   PASS 0: one tile = the whole polynomial (n <= 2^TB), all stages;
+//   PASS 1: 2^k1 rows x 2^cb columns (row stride 2^k2), the top k1 stages;
+//   PASS 2: 2^rb rows x 2^k2 contiguous words, the low k2 stages.
+// Everything below is fixed at compile time by (LOGN, PASS, TB), TB being
+// the tile size (log2 words) of the two-pass path.
+// Pass A takes the top floor(log2 n / 2) stages (0 for one-tile sizes).
+constexpr int ntt_k1(int logn, int tile_bits) {
+	return logn <= tile_bits ? 0 : logn / 2;
+}
+
+template <int LOGN, int PASS, int TB>
+struct Geo {
+	static constexpr int K1 = ntt_k1(LOGN, TB);
+	static constexpr int K2 = LOGN - K1;
+	static constexpr int KT = PASS == 0 ? LOGN : TB;
+	static constexpr int CB = TB - K1;  // pass 1 columns per tile
+	static constexpr int LO = PASS == 1 ? CB : 0;
+	static constexpr int HI = PASS == 0 ? LOGN : PASS == 1 ? TB : K2;
+	// low local bits contiguous in memory, and the stride of the rest
+	static constexpr int CBITS = PASS == 1 ? CB : KT;
+	static constexpr int JSHIFT = PASS == 1 ? K2 : 0;
+	// global stage bit = local bit + SHIFT
+	static constexpr int SHIFT = PASS == 1 ? K2 - CB : 0;
+	static constexpr int ROUNDS = (HI - LO + 2) / 3;
+	static constexpr int THREADS = 1 << (KT - 3);
+	static constexpr int NTILES = PASS == 0 ? 1 : 1 << (LOGN - TB);
+};
+
+// Field position and stage range of round R (forward: top-down; inverse:
+// bottom-up).
+template <class G, bool INV, int R>
+struct Round {
+	static constexpr int BHI_F = G::HI - 3 * R;
+	static constexpr int BLO_F = BHI_F - 3 >= G::LO ? BHI_F - 3 : G::LO;
+	static constexpr int BLO_I = G::LO + 3 * R;
+	static constexpr int BHI_I = BLO_I + 3 <= G::HI ? BLO_I + 3 : G::HI;
+	static constexpr int BLO = INV ? BLO_I : BLO_F;
+	static constexpr int BHI = INV ? BHI_I : BHI_F;
+	static constexpr int F = BLO + 3 <= G::KT ? BLO : G::KT - 3;
+	// Lanes of a wave are contiguous in memory for 2^RUN words; direct
+	// global access is used when that covers a 128-byte line.
+	static constexpr int RUN = F < G::CBITS ? F : G::CBITS;
+	static constexpr bool DIRECT = RUN >= 4;
+};
+
+// One padding word per 8 keeps the radix-8 access patterns conflict free.
+__device__ __forceinline__ uint32_t pad(uint32_t l) {
+	return l + (l >> 3);
+}
+
+template <class G>
+__device__ __forceinline__ uint32_t tile_addr(uint32_t l, uint32_t jbase) {
+	if (G::CBITS >= G::KT)
+		return jbase + l;
+	return ((l >> G::CBITS) << G::JSHIFT) + (l & ((1u << G::CBITS) - 1)) +
+		jbase;
+}
+
+// Local index of element 0 of thread t in a round whose 3-bit field sits at
+// local bit f; element e is at l0 + (e << f).
+template <int F>
+__device__ __forceinline__ uint32_t round_base(uint32_t t) {
+	return ((t >> F) << (F + 3)) | (t & ((1u << F) - 1));
+}
+
+// Twiddle slots of one round: bit 2 of the field uses slot 0, bit 1 slots
+// 1-2, bit 0 slots 3-6.
+__host__ __device__ constexpr int tw_slot(int d, int gi) {
+	return (d == 2 ? 0 : d == 1 ? 1 : 3) + gi;
+}
+
+template <class G, bool INV, int R, int LOGN>
+__device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
+		const uint64_t *src, uint64_t *dst, uint64_t *lds, uint32_t jbase,
+		uint32_t gbase, const ulonglong2 *tw, const NttConsts &c, bool final) {
+	using RD = Round<G, INV, R>;
+	constexpr int F = RD::F;
+	constexpr bool FIRST_ROUND = R == 0;
+	constexpr bool LAST_ROUND = R == G::ROUNDS - 1;
+	const uint32_t l0 = round_base<F>(t);
+	const bool active = t < (uint32_t) G::THREADS;
+
+	// ---- operands ----
+	if (FIRST_ROUND && !RD::DIRECT) {
+		// lane-contiguous global loads staged through LDS
+		if (active) {
+#pragma unroll
+			for (int k = 0; k < 8; k++) {
+				const uint32_t l = t + k * G::THREADS;
+				lds[pad(l)] = src[tile_addr<G>(l, jbase)];
+			}
+		}
+		__syncthreads();
+	}
+	ulonglong2 w[7];
+	if (active) {
+#pragma unroll
+		for (int e = 0; e < 8; e++) {
+			const uint32_t l = l0 + (e << F);
+			if (FIRST_ROUND && RD::DIRECT)
+				v[e] = src[tile_addr<G>(l, jbase)];
+			else
+				v[e] = lds[pad(l)];
+		}
+		// every twiddle of the round, issued before any arithmetic
+#pragma unroll
+		for (int d = 2; d >= 0; d--) {
+			const int lb = F + d;
+			if (lb < RD::BLO || lb >= RD::BHI)
+				continue;
+			const int b = lb + G::SHIFT;
+			const uint32_t ti = (1u << (LOGN - 1 - b)) + ((gbase + l0) >> (lb + 1));
+#pragma unroll
+			for (int gi = 0; gi < (4 >> d); gi++)
+				w[tw_slot(d, gi)] = tw[ti + gi];
+		}
+	}
+
+	// ---- butterflies ----
+	if (active) {
+		const uint64_t nq = 0 - c.q;
+#pragma unroll
+		for (int k = 0; k < 3; k++) {
+			const int d = INV ? k : 2 - k;
+			const int lb = F + d;
+			if (lb < RD::BLO || lb >= RD::BHI)
+				continue;
+			const int b = lb + G::SHIFT;
+			if (!INV && b == LOGN - 1) {
+#pragma unroll
+				for (int e = 0; e < 4; e++)
+					bf_fwd_first(v[e], v[e + 4], w[0], c);
+			} else if (INV && b == LOGN - 1) {
+#pragma unroll
+				for (int e = 0; e < 4; e++)
+					bf_inv_last(v[e], v[e + 4], c);
+			} else {
+#pragma unroll
+				for (int gi = 0; gi < (4 >> d); gi++) {
+#pragma unroll
+					for (int li = 0; li < (1 << d); li++) {
+						const int e = (gi << (d + 1)) | li;
+						if (INV)
+							bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
+									c.two_q, nq);
+						else
+							bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
+									c.two_q, nq);
+					}
+				}
+			}
+		}
+		if (!INV && LAST_ROUND && final) {
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+				v[e] = csub(csub(v[e], c.two_q), c.q);
+		}
+	}
+
+	// ---- results ----
+	if (LAST_ROUND && RD::DIRECT) {
+		if (active) {
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+				dst[tile_addr<G>(l0 + (e << F), jbase)] = v[e];
+		}
+		__syncthreads();
+	} else {
+		if (active) {
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+				lds[pad(l0 + (e << F))] = v[e];
+		}
+		__syncthreads();
+		if (LAST_ROUND) {
+			if (active) {
+#pragma unroll
+				for (int k = 0; k < 8; k++) {
+					const uint32_t l = t + k * G::THREADS;
+					dst[tile_addr<G>(l, jbase)] = lds[pad(l)];
+				}
+			}
+			__syncthreads();
+		}
+	}
+}
+
+template <class G, bool INV, int LOGN, int... R>
+__device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...>,
+		uint32_t t, const uint64_t *src, uint64_t *dst, uint64_t *lds,
+		uint32_t jbase, uint32_t gbase, const ulonglong2 *tw, const NttConsts &c,
+		bool final) {
+	uint64_t v[8];
+	(run_round<G, INV, R, LOGN>(v, t, src, dst, lds, jbase, gbase, tw, c,
+			final), ...);
+}
+
+template <int LOGN, int PASS, int TB, bool INV>
+__device__ __forceinline__ void run_pass(const uint64_t *src, uint64_t *dst,
+		uint64_t *lds, const ulonglong2 *tw, const NttConsts &c, bool final) {
+	using G = Geo<LOGN, PASS, TB>;
+	const uint32_t t = threadIdx.x;
+	for (uint32_t i = 0; i < (uint32_t) G::NTILES; i++) {
+		// pass 1 tiles start at column i * 2^cb; pass 2 tiles at row block
+		// i (2^TB contiguous words), which is also their twiddle offset.
+		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
+		const uint32_t gbase = PASS == 2 ? jbase : 0;
+		run_rounds<G, INV, LOGN>(std::make_integer_sequence<int, G::ROUNDS>(), t,
+				src, dst, lds, jbase, gbase, tw, c, final);
+	}
+}
+
+template <int LOGN, int TB>
+struct FusedShape {
+	static constexpr int KT = LOGN <= TB ? LOGN : TB;
+	static constexpr int THREADS = (1 << (KT - 3)) < 64 ? 64 : 1 << (KT - 3);
+	static constexpr int LDS_WORDS = (1 << KT) + (1 << (KT - 3));
+};
+
+// One workgroup per task.  With `tasks` null, workgroup i transforms the
+// i-th polynomial after `uniform` (same tables, contiguous polynomials).
+template <int LOGN, bool INV, int TB>
+__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
+		NttTask uniform, const NttTask *tasks) {
+	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
+	NttTask k;
+	if (tasks) {
+		k = tasks[blockIdx.x];
+	} else {
+		k = uniform;
+		const uint64_t off = (uint64_t) blockIdx.x << LOGN;
+		k.in += off;
+		k.out += off;
+	}
+	if constexpr (LOGN <= TB) {
+		run_pass<LOGN, 0, TB, INV>(k.in, k.out, lds, k.tw, k.c, true);
+	} else if constexpr (!INV) {
+		run_pass<LOGN, 1, TB, INV>(k.in, k.out, lds, k.tw, k.c, false);
+		run_pass<LOGN, 2, TB, INV>(k.out, k.out, lds, k.tw, k.c, true);
+	} else {
+		run_pass<LOGN, 2, TB, INV>(k.in, k.out, lds, k.tw, k.c, false);
+		run_pass<LOGN, 1, TB, INV>(k.out, k.out, lds, k.tw, k.c, true);
+	}
+}
+
+// One pass of a two-pass size, one workgroup per (tile, polynomial):
+// blockIdx.x = tile, blockIdx.y = polynomial.  The first pass of a
+// direction reads `in`, the second reads and writes `out`.
+template <int LOGN, int PASS, bool INV, int TB>
+__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_pass(
+		NttTask uniform, const NttTask *tasks, uint32_t second) {
+	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
+	if constexpr (LOGN > TB) {
+		NttTask k;
+		if (tasks) {
+			k = tasks[blockIdx.y];
+		} else {
+			k = uniform;
+			const uint64_t off = (uint64_t) blockIdx.y << LOGN;
+			k.in += off;
+			k.out += off;
+		}
+		using G = Geo<LOGN, PASS, TB>;
+		const uint32_t i = blockIdx.x;
+		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
+		const uint32_t gbase = PASS == 2 ? jbase : 0;
+		run_rounds<G, INV, LOGN>(std::make_integer_sequence<int, G::ROUNDS>(),
+				threadIdx.x, second ? k.out : k.in, k.out, lds, jbase, gbase, k.tw,
+				k.c, second != 0);
+	}
+}
+
+// Kernel table for tile size TB, entry i serving log2 n = i + 3.
+template <int TB, int... L>
+constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
+		std::integer_sequence<int, L...>) {
+	return {FusedEntry{k_ntt_fused<L + kMinFusedLogn, false, TB>,
+		k_ntt_fused<L + kMinFusedLogn, true, TB>,
+		k_ntt_pass<L + kMinFusedLogn, 1, false, TB>,
+		k_ntt_pass<L + kMinFusedLogn, 2, false, TB>,
+		k_ntt_pass<L + kMinFusedLogn, 2, true, TB>,
+		k_ntt_pass<L + kMinFusedLogn, 1, true, TB>,
+		FusedShape<L + kMinFusedLogn, TB>::THREADS,
+		L + kMinFusedLogn > TB ? 1 << (L + kMinFusedLogn - TB) : 1}...};
+}
+
+// Per-tile-size tables (ntt_fused_kt*.hip).  Sizes n <= 2^TB use the
+// one-tile kernel and are identical across tables.
+const FusedEntry *fused_table_kt11();
+const FusedEntry *fused_table_kt12();
+const FusedEntry *fused_table_kt13();
+
+}  // namespace vk

@@ -59,6 +59,11 @@ SIGNATURES = {
     "vkhel_ext_tables_copy": (_int, [_p, _pu64, _pu64, _pu64, _pu64]),
     "vkhel_ext_tables_upload": (_int, [_p, _p]),
     "vkhel_ext_set_ntt_algo": (_int, [_p, _int]),
+    "vkhel_ext_autotune": (_int, [_p, _u32, _u32, _u32]),
+    "vkhel_ext_tile_bits": (_int, [_p, _u32]),
+    "vkhel_ext_set_tile_bits": (_int, [_p, _u32, _int]),
+    "vkhel_ext_autotune_save": (_int, [_p, ctypes.c_char_p]),
+    "vkhel_ext_autotune_load": (_int, [_p, ctypes.c_char_p]),
     "vkhel_batch_create": (_p, [_p]),
     "vkhel_batch_destroy": (None, [_p]),
     "vkhel_batch_clear": (None, [_p]),

@@ -52,6 +52,24 @@ class Context:
     def set_ntt_algo(self, algo):
         _lib.check(_L().vkhel_ext_set_ntt_algo(self._h, algo))
 
+    def autotune(self, logn_lo=12, logn_hi=17, batch=0):
+        """Times each compiled tile size per transform size; keeps the best."""
+        _lib.check(_L().vkhel_ext_autotune(self._h, logn_lo, logn_hi, batch))
+
+    def tile_bits(self, logn):
+        tb = _L().vkhel_ext_tile_bits(self._h, logn)
+        _lib.check()
+        return tb
+
+    def set_tile_bits(self, logn, tile_bits):
+        _lib.check(_L().vkhel_ext_set_tile_bits(self._h, logn, tile_bits))
+
+    def autotune_save(self, path):
+        _lib.check(_L().vkhel_ext_autotune_save(self._h, str(path).encode()))
+
+    def autotune_load(self, path):
+        _lib.check(_L().vkhel_ext_autotune_load(self._h, str(path).encode()))
+
     def vector(self, length, zero=True):
         return Vector(self, length, zero)
 
