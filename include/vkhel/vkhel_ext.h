@@ -54,6 +54,11 @@ int vkhel_ext_vector_read(const struct vkhel_vector *, uint64_t offset,
 int vkhel_ext_vector_write(struct vkhel_vector *, uint64_t offset,
 		uint64_t count, const uint64_t *host);
 
+/* Fills `count` words from `offset` with synthetic residues uniform in
+ * [0, q): word i is the high half of splitmix64(seed, i) * q. */
+int vkhel_ext_vector_fill_uniform(struct vkhel_vector *, uint64_t offset,
+		uint64_t count, uint64_t q, uint64_t seed);
+
 /* ---- tables ---- */
 /* Reads back the reference-layout host tables (n words each). */
 int vkhel_ext_tables_info(const struct vkhel_ntt_tables *, uint64_t *n,
@@ -119,6 +124,82 @@ int vkhel_batch_launch(struct vkhel_batch *);
 /* vkhel_batch_launch + wait. */
 int vkhel_batch_run(struct vkhel_batch *);
 uint64_t vkhel_batch_num_jobs(const struct vkhel_batch *);
+
+/* ---- serving engine ----
+ * Continuous batching over device-resident operands.  Requests use the
+ * batch job model.  They are queued on submit and admitted in arrival order
+ * while the device bytes of admitted, unfinished requests (operands, result
+ * and polymul scratch) stay within hbm_budget_bytes and at most
+ * max_inflight requests are in flight.  `streams` workers each own a HIP
+ * stream; a free worker takes every admissible waiting request (up to
+ * max_batch) as one micro-batch, so requests arriving while others run are
+ * admitted as budget and slots free up and overlap with them on the device.
+ * A request's result does not depend on what it was batched with.  Callers
+ * keep operands alive and unmodified until the request completes.
+ */
+struct vkhel_server_config {
+	uint64_t hbm_budget_bytes; /* 0: 200 GB */
+	uint32_t max_inflight;     /* 0: 4096 requests */
+	uint32_t max_batch;        /* requests per micro-batch, 0: 256 */
+	uint32_t streams;          /* worker streams, 0: 4 (at most 32) */
+	uint32_t queue_capacity;   /* waiting requests, 0: 65536 */
+};
+
+struct vkhel_request_info {
+	int status;            /* VKHEL_OK, VKHEL_ERR_CANCELLED, ... */
+	uint64_t submit_ns;    /* steady-clock timestamps */
+	uint64_t admit_ns;
+	uint64_t complete_ns;
+	uint32_t batch_size;   /* requests in the micro-batch that ran it */
+};
+
+struct vkhel_server_stats {
+	uint64_t submitted, completed, cancelled, rejected;
+	uint64_t batches;            /* micro-batches run */
+	uint64_t transform_coeffs;   /* NTT coefficients processed */
+	uint64_t inflight_bytes, peak_inflight_bytes;
+	uint32_t inflight, waiting;
+};
+
+struct vkhel_server;
+struct vkhel_server *vkhel_server_create(struct vkhel_ctx *,
+		const struct vkhel_server_config *);
+/* Finishes admitted and waiting work, then stops the workers. */
+void vkhel_server_destroy(struct vkhel_server *);
+/* Returns a request id > 0, or 0 with the error recorded: VKHEL_ERR_BUDGET
+ * if the request alone exceeds the budget, VKHEL_ERR_BUSY if the queue is
+ * full, VKHEL_ERR_INVALID for bad arguments. */
+uint64_t vkhel_server_submit(struct vkhel_server *, int op,
+		const struct vkhel_vector *a, const struct vkhel_vector *b,
+		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,
+		uint32_t nlimbs);
+/* Waits up to timeout_ns (0: no limit).  Returns the request's status, or
+ * VKHEL_ERR_TIMEOUT if it is still pending.  A completed request is
+ * released by the wait that returns its status. */
+int vkhel_server_wait(struct vkhel_server *, uint64_t id, uint64_t timeout_ns,
+		struct vkhel_request_info *info);
+/* Cancels a request that has not been admitted yet. */
+int vkhel_server_cancel(struct vkhel_server *, uint64_t id);
+/* Blocks until nothing is waiting or in flight. */
+int vkhel_server_drain(struct vkhel_server *);
+int vkhel_server_get_stats(struct vkhel_server *, struct vkhel_server_stats *);
+
+/* Load generator: submits jobs[i] at the i-th arrival of a Poisson process
+ * of `rate_per_s` arrivals per second (exponential gaps from a seeded
+ * splitmix64 stream; rate 0 submits everything at once), waits for all of
+ * them and stores each request's record in infos[i].  Returns VKHEL_OK or
+ * the first failing status. */
+struct vkhel_job_desc {
+	int op;
+	const struct vkhel_vector *a;
+	const struct vkhel_vector *b;
+	struct vkhel_vector *res;
+	struct vkhel_ntt_tables *const *tables;
+	uint32_t nlimbs;
+};
+int vkhel_server_replay(struct vkhel_server *, const struct vkhel_job_desc *jobs,
+		uint64_t njobs, double rate_per_s, uint64_t seed,
+		struct vkhel_request_info *infos);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,179 @@
+"""GPU tests of the serving engine (vkhel_server_*, vkhel_ext.h).
+
+Results must be bit-identical whether a request runs alone or mixed with
+others, and equal to the oracle.  Admission, budget, queue, cancel and
+timeout behaviour are forced from the host (budgets, queue sizes); no test
+makes a kernel misbehave.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import vkhel_amd as V
+from vkhel_amd import vkhel as VV
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = V.Context()
+    yield c
+    c.close()
+
+
+_TABLE_CACHE = {}
+
+
+def tables_for(logn, nlimbs):
+    n = 1 << logn
+    key = (logn, nlimbs)
+    if key not in _TABLE_CACHE:
+        qs = O.primes_ntt(1 << 60, 1 << 18, nlimbs)
+        _TABLE_CACHE[key] = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+    return _TABLE_CACHE[key]
+
+
+def make_jobs(ctx, rng, count, logn_range=(10, 15), max_limbs=4):
+    jobs = []
+    for k in range(count):
+        logn = int(rng.integers(*logn_range))
+        L = int(rng.integers(1, max_limbs + 1))
+        tabs = tables_for(logn, L)
+        n = 1 << logn
+        op = int(rng.integers(0, 5))
+        a, b, r = ctx.vector(L * n, False), ctx.vector(L * n, False), ctx.vector(L * n)
+        for i, t in enumerate(tabs):
+            a.fill_uniform(t.q, 1000 * k + i, i * n, n)
+            b.fill_uniform(t.q, 1000 * k + 500 + i, i * n, n)
+        jobs.append((op, a, b if op >= V.JOB_POLYMUL else None, r, tabs))
+    return jobs
+
+
+def run_alone(ctx, job):
+    op, a, b, r, tabs = job
+    alone = ctx.vector(r.length)
+    bt = V.Batch(ctx)
+    bt.add(op, a, b, alone, tabs)
+    bt.run()
+    out = alone.read()
+    bt.close()
+    alone.close()
+    return out
+
+
+def test_served_results_equal_alone_and_oracle(ctx):
+    rng = np.random.default_rng(11)
+    jobs = make_jobs(ctx, rng, 60)
+    srv = V.Server(ctx, streams=4, max_batch=16)
+    st, infos = srv.replay(jobs, rate_per_s=0)
+    assert st == VV.OK
+    assert all(i.status == VV.OK for i in infos)
+    assert max(i.batch_size for i in infos) > 1  # requests were batched together
+    for k, job in enumerate(jobs):
+        op, a, b, r, tabs = job
+        got = r.read()
+        assert np.array_equal(got, run_alone(ctx, job)), k
+        if k % 6 == 0:  # oracle spot checks across ops
+            n = tabs[0].n
+            xa = a.read()
+            xb = b.read() if b is not None else None
+            for i, t in enumerate(tabs):
+                sl = slice(i * n, (i + 1) * n)
+                if op == V.JOB_NTT_FWD:
+                    want = O.ntt_forward(xa[sl], t.q, t.w)
+                elif op == V.JOB_NTT_INV:
+                    want = O.ntt_inverse(xa[sl], t.q, t.w)
+                elif op == V.JOB_POLYMUL:
+                    want = O.polymul_ntt(xa[sl], xb[sl], t.q, t.w)
+                elif op == V.JOB_MUL:
+                    want = O.elemmul(xa[sl], xb[sl], t.q)
+                else:
+                    want = O.addmul_mod(xa[sl], xb[sl], 1, t.q)
+                assert np.array_equal(got[sl], want), (k, op, i)
+    stats = srv.stats()
+    assert stats.completed == len(jobs) and stats.inflight == 0
+    srv.close()
+    for _, a, b, r, _ in jobs:
+        a.close()
+        r.close()
+        if b is not None:
+            b.close()
+
+
+def test_poisson_arrivals_latency_records(ctx):
+    rng = np.random.default_rng(12)
+    jobs = make_jobs(ctx, rng, 80, (10, 13), 3)
+    srv = V.Server(ctx, streams=2)
+    st, infos = srv.replay(jobs, rate_per_s=20000, seed=5)
+    assert st == VV.OK
+    for i in infos:
+        assert i.status == VV.OK
+        assert i.submit_ns <= i.admit_ns <= i.complete_ns
+    subs = [i.submit_ns for i in infos]
+    assert subs == sorted(subs)
+    srv.close()
+
+
+def test_budget_limits_inflight_bytes(ctx):
+    rng = np.random.default_rng(13)
+    jobs = make_jobs(ctx, rng, 24, (12, 14), 2)
+    # admission cost: operands + result (+ the spectrum of b for polymul)
+    def cost(op, b, words):
+        return 8 * words * ((3 if b is not None else 2) + (op == V.JOB_POLYMUL))
+    biggest = max(cost(op, b, r.length) for op, a, b, r, tabs in jobs)
+    budget = int(biggest * 2)
+    srv = V.Server(ctx, hbm_budget_bytes=budget, streams=4)
+    st, infos = srv.replay(jobs, 0)
+    assert st == VV.OK
+    s = srv.stats()
+    assert 0 < s.peak_inflight_bytes <= budget
+    srv.close()
+
+
+def test_request_over_budget_rejected(ctx):
+    tabs = tables_for(12, 1)
+    a, r = ctx.vector(4096), ctx.vector(4096)
+    srv = V.Server(ctx, hbm_budget_bytes=1000)
+    with pytest.raises(V.VkhelError) as e:
+        srv.submit(V.JOB_NTT_FWD, a, None, r, tabs)
+    assert e.value.code == VV.ERR_BUDGET
+    assert srv.stats().rejected == 1
+    srv.close()
+
+
+def test_queue_full_cancel_and_timeout(ctx):
+    # One stream, one request in flight at a time, big polymul requests so
+    # later ones are still waiting while we act on them.
+    tabs = tables_for(16, 8)
+    n = 1 << 16
+    vecs, ops = [], []
+    for k in range(8):
+        a, b, r = (ctx.vector(8 * n, False) for _ in range(3))
+        for i, t in enumerate(tabs):
+            a.fill_uniform(t.q, k * 10 + i, i * n, n)
+            b.fill_uniform(t.q, k * 10 + 5 + i, i * n, n)
+        vecs += [a, b, r]
+        ops.append((a, b, r))
+    srv = V.Server(ctx, streams=1, max_inflight=1, queue_capacity=3)
+    ids, busy = [], 0
+    for a, b, r in ops:
+        try:
+            ids.append(srv.submit(V.JOB_POLYMUL, a, b, r, tabs))
+        except V.VkhelError as e:
+            assert e.code == VV.ERR_BUSY
+            busy += 1
+    assert busy >= 1
+    last = ids[-1]
+    st, _ = srv.wait(last, timeout_ns=1)
+    assert st == VV.ERR_TIMEOUT
+    srv.cancel(last)
+    st, info = srv.wait(last)
+    assert st == VV.ERR_CANCELLED and info.status == VV.ERR_CANCELLED
+    for rid in ids[:-1]:
+        st, _ = srv.wait(rid)
+        assert st == VV.OK
+    assert srv.stats().cancelled == 1
+    srv.close()
+    for v in vecs:
+        v.close()

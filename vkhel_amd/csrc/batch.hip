@@ -2,37 +2,24 @@
 // reference has no batch API: a caller would issue one synchronous
 // vkhel_vector_* call per polynomial (src/vector.c:513-657), each paying
 // log2(n) fenced submits.  Here a job list turns into three device phases,
-// each one launch over every polynomial limb of every job:
+// each a handful of launches over every polynomial limb of every job:
 //
 //   1. forward transforms   (NTT_FWD jobs, both operands of POLYMUL jobs)
 //   2. pointwise products   (MUL, ADD jobs, the spectra of POLYMUL jobs)
 //   3. inverse transforms   (NTT_INV jobs, POLYMUL results)
 //
-// The launch sequence is captured once into a hipGraph and replayed while
-// the job list is unchanged.
+// Transform tasks are grouped by size: one launch per distinct n, each
+// workgroup taking one polynomial limb.  The batch API captures the launch
+// sequence once into a hipGraph and replays it while the job list is
+// unchanged.
 #include <algorithm>
 #include <vector>
 
-#include "internal.h"
+#include "jobs.h"
 #include "modarith.h"
 #include "numbers.h"
 
 namespace vk {
-
-struct Job {
-	int op;
-	const vkhel_vector *a, *b;
-	vkhel_vector *res;
-	std::vector<const vkhel_ntt_tables *> tables;
-};
-
-// One pointwise limb: out = a (op) b over len words mod q.
-struct PointTask {
-	const uint64_t *a, *b;
-	uint64_t *out;
-	uint64_t len, q, mu;
-	uint32_t shift, op;  // op: 0 = mul (elemmul semantics), 1 = add
-};
 
 __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks) {
 	const PointTask t = tasks[blockIdx.y];
@@ -68,56 +55,68 @@ __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks)
 	}
 }
 
-}  // namespace vk
-
-namespace vk {
-// A run of tasks with one log2 n inside a direction's task array.
-struct Segment {
-	uint32_t off, count, logn;
-};
-}  // namespace vk
-
-struct vkhel_batch {
-	vkhel_ctx *ctx;
-	std::vector<vk::Job> jobs;
-	bool dirty = true;
-	// Device state built at launch.
-	vk::NttTask *d_fwd = nullptr, *d_inv = nullptr;
-	vk::PointTask *d_pt = nullptr;
-	std::vector<vk::NttTask> h_fwd, h_inv;
-	std::vector<vk::Segment> seg_fwd, seg_inv;
-	uint32_t n_pt = 0;
-	uint64_t pt_maxlen = 0;
-	uint64_t *scratch = nullptr;
-	size_t scratch_words = 0;
-	hipGraph_t graph = nullptr;
-	hipGraphExec_t exec = nullptr;
-};
-
-namespace vk {
-
-static void release_device(vkhel_batch *b) {
-	if (b->exec)
-		(void) hipGraphExecDestroy(b->exec);
-	if (b->graph)
-		(void) hipGraphDestroy(b->graph);
-	(void) hipFree(b->d_fwd);
-	(void) hipFree(b->d_inv);
-	(void) hipFree(b->d_pt);
-	b->exec = nullptr;
-	b->graph = nullptr;
-	b->d_fwd = b->d_inv = nullptr;
-	b->d_pt = nullptr;
+void DeviceTasks::release() {
+	(void) hipFree(fwd);
+	(void) hipFree(inv);
+	(void) hipFree(pt);
+	fwd = inv = nullptr;
+	pt = nullptr;
+	cap_fwd = cap_inv = cap_pt = 0;
 }
 
-template <class T>
-static bool upload(const std::vector<T> &h, T **d) {
-	if (h.empty())
-		return true;
-	if (!hip_ok(hipMalloc(d, h.size() * sizeof(T)), "hipMalloc(batch tasks)"))
-		return false;
-	return hip_ok(hipMemcpy(*d, h.data(), h.size() * sizeof(T),
-				hipMemcpyHostToDevice), "upload batch tasks");
+int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
+		const vkhel_vector *b, vkhel_vector *res,
+		vkhel_ntt_tables *const *tables, uint32_t nlimbs, Job *out) {
+	if (!a || !res || !tables || nlimbs == 0)
+		return fail(VKHEL_ERR_INVALID, "job: null argument");
+	if (op < VKHEL_JOB_NTT_FWD || op > VKHEL_JOB_ADD)
+		return fail(VKHEL_ERR_INVALID, "job: unknown op %d", op);
+	const bool binary = op >= VKHEL_JOB_POLYMUL;
+	if (binary && !b)
+		return fail(VKHEL_ERR_INVALID, "job: op needs a second operand");
+	if (a->ctx != ctx || res->ctx != ctx || (binary && b->ctx != ctx))
+		return fail(VKHEL_ERR_INVALID, "job: vector from another context");
+	for (uint32_t i = 0; i < nlimbs; i++)
+		if (!tables[i] || tables[i]->n != tables[0]->n)
+			return fail(VKHEL_ERR_INVALID,
+					"job: limb tables must all exist and share n");
+	const uint64_t words = (uint64_t) nlimbs * tables[0]->n;
+	if (a->length < words || res->length < words ||
+			(binary && b->length < words))
+		return fail(VKHEL_ERR_INVALID,
+				"job: vectors need %llu words (%u limbs x n)",
+				(unsigned long long) words, nlimbs);
+	if (op == VKHEL_JOB_POLYMUL && b == res)
+		return fail(VKHEL_ERR_INVALID, "job: polymul result aliases b");
+	out->op = op;
+	out->a = a;
+	out->b = binary ? b : nullptr;
+	out->res = res;
+	out->tables.assign(tables, tables + nlimbs);
+	return VKHEL_OK;
+}
+
+uint64_t job_scratch_words(const Job &j) {
+	return j.op == VKHEL_JOB_POLYMUL ? j.tables.size() * j.tables[0]->n : 0;
+}
+
+uint64_t job_bytes(const Job &j) {
+	const uint64_t words = j.tables.size() * j.tables[0]->n;
+	const uint64_t operands = j.b ? 3 : 2;
+	return 8 * (operands * words + job_scratch_words(j));
+}
+
+uint64_t job_transform_coeffs(const Job &j) {
+	const uint64_t words = j.tables.size() * j.tables[0]->n;
+	switch (j.op) {
+	case VKHEL_JOB_POLYMUL:
+		return 3 * words;  // two forward transforms and one inverse
+	case VKHEL_JOB_NTT_FWD:
+	case VKHEL_JOB_NTT_INV:
+		return words;
+	default:
+		return 0;
+	}
 }
 
 // Sorts tasks by log2 n and records the runs.
@@ -131,6 +130,87 @@ static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
 		segs.back().count++;
 	}
 	return segs;
+}
+
+int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
+		uint64_t *scratch, TaskSet &ts) {
+	ts = TaskSet();
+	uint64_t *sp = scratch;
+	for (size_t ji = 0; ji < njobs; ji++) {
+		const Job &j = *jobs[ji];
+		const uint64_t n = j.tables[0]->n;
+		for (size_t l = 0; l < j.tables.size(); l++) {
+			const DevTables *t = device_tables(ctx, j.tables[l]);
+			if (!t)
+				return vkhel_ext_last_error();
+			const uint64_t off = l * n;
+			const uint64_t q = t->c.q;
+			const PointTask mul{nullptr, nullptr, nullptr, n, q, barrett_mu(q),
+				(uint32_t) (bitlen(q) - 2), 0};
+			switch (j.op) {
+			case VKHEL_JOB_NTT_FWD:
+				ts.fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
+				break;
+			case VKHEL_JOB_NTT_INV:
+				ts.inv.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, true));
+				break;
+			case VKHEL_JOB_POLYMUL: {
+				uint64_t *tmp = sp;
+				sp += n;
+				ts.fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
+				ts.fwd.push_back(make_task(*t, j.b->dev + off, tmp, false));
+				PointTask p = mul;
+				p.a = j.res->dev + off;
+				p.b = tmp;
+				p.out = j.res->dev + off;
+				ts.pt.push_back(p);
+				ts.inv.push_back(make_task(*t, j.res->dev + off, j.res->dev + off, true));
+				break;
+			}
+			case VKHEL_JOB_MUL:
+			case VKHEL_JOB_ADD: {
+				PointTask p = mul;
+				p.a = j.a->dev + off;
+				p.b = j.b->dev + off;
+				p.out = j.res->dev + off;
+				p.op = j.op == VKHEL_JOB_MUL ? 0 : 1;
+				ts.pt.push_back(p);
+				break;
+			}
+			}
+			if (j.op >= VKHEL_JOB_POLYMUL)
+				ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
+		}
+	}
+	ts.seg_fwd = segment(ts.fwd);
+	ts.seg_inv = segment(ts.inv);
+	return VKHEL_OK;
+}
+
+template <class T>
+static bool upload_array(const std::vector<T> &h, T **d, size_t *cap,
+		hipStream_t s) {
+	if (h.empty())
+		return true;
+	if (h.size() > *cap) {
+		const size_t want = std::max<size_t>(h.size(), 2 * *cap);
+		(void) hipFree(*d);
+		*d = nullptr;
+		*cap = 0;
+		if (!hip_ok(hipMalloc(d, want * sizeof(T)), "hipMalloc(task array)"))
+			return false;
+		*cap = want;
+	}
+	return hip_ok(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T),
+				hipMemcpyHostToDevice, s), "upload task array");
+}
+
+int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s) {
+	if (!upload_array(ts.fwd, &dt.fwd, &dt.cap_fwd, s) ||
+			!upload_array(ts.inv, &dt.inv, &dt.cap_inv, s) ||
+			!upload_array(ts.pt, &dt.pt, &dt.cap_pt, s))
+		return vkhel_ext_last_error();
+	return VKHEL_OK;
 }
 
 static int launch_segments(const vkhel_ctx *ctx,
@@ -150,32 +230,54 @@ static int launch_segments(const vkhel_ctx *ctx,
 	return VKHEL_OK;
 }
 
-static int enqueue(vkhel_batch *b, hipStream_t s) {
-	int rc = launch_segments(b->ctx, b->seg_fwd, b->h_fwd, b->d_fwd, false, s);
-	if (rc == VKHEL_OK && b->n_pt) {
-		const uint64_t blocks = std::min<uint64_t>((b->pt_maxlen / 2 + 255) / 256,
+int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
+		const DeviceTasks &dt, hipStream_t s) {
+	int rc = launch_segments(ctx, ts.seg_fwd, ts.fwd, dt.fwd, false, s);
+	if (rc == VKHEL_OK && !ts.pt.empty()) {
+		const uint64_t blocks = std::min<uint64_t>((ts.pt_maxlen / 2 + 255) / 256,
 				64);
 		hipLaunchKernelGGL(k_pointwise_tasks, dim3((unsigned) std::max<uint64_t>(
-						blocks, 1), b->n_pt), dim3(256), 0, s, b->d_pt);
+						blocks, 1), (unsigned) ts.pt.size()), dim3(256), 0, s, dt.pt);
 		rc = hip_ok(hipGetLastError(), "pointwise launch") ? VKHEL_OK
 			: VKHEL_ERR_DEVICE;
 	}
 	if (rc == VKHEL_OK)
-		rc = launch_segments(b->ctx, b->seg_inv, b->h_inv, b->d_inv, true, s);
+		rc = launch_segments(ctx, ts.seg_inv, ts.inv, dt.inv, true, s);
 	return rc;
+}
+
+}  // namespace vk
+
+struct vkhel_batch {
+	vkhel_ctx *ctx;
+	std::vector<vk::Job> jobs;
+	bool dirty = true;
+	vk::TaskSet ts;
+	vk::DeviceTasks dt;
+	uint64_t *scratch = nullptr;
+	size_t scratch_words = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+};
+
+namespace vk {
+
+static void release_graph(vkhel_batch *b) {
+	if (b->exec)
+		(void) hipGraphExecDestroy(b->exec);
+	if (b->graph)
+		(void) hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
 }
 
 // Turns the job list into device task arrays and a captured graph.
 static int build(vkhel_batch *b) {
-	release_device(b);
+	release_graph(b);
 	vkhel_ctx *ctx = b->ctx;
-	std::vector<NttTask> fwd, inv;
-	std::vector<PointTask> pt;
-	uint64_t pt_maxlen = 0;
-	size_t scratch_words = 0;
+	uint64_t scratch_words = 0;
 	for (const Job &j : b->jobs)
-		if (j.op == VKHEL_JOB_POLYMUL)
-			scratch_words += j.tables.size() * j.tables[0]->n;
+		scratch_words += job_scratch_words(j);
 	if (scratch_words > b->scratch_words) {
 		(void) hipFree(b->scratch);
 		b->scratch = nullptr;
@@ -185,59 +287,24 @@ static int build(vkhel_batch *b) {
 			return vkhel_ext_last_error();
 		b->scratch_words = scratch_words;
 	}
-	uint64_t *sp = b->scratch;
-	for (const Job &j : b->jobs) {
-		const uint64_t n = j.tables[0]->n;
-		for (size_t l = 0; l < j.tables.size(); l++) {
-			const DevTables *t = device_tables(ctx, j.tables[l]);
-			if (!t)
-				return vkhel_ext_last_error();
-			const uint64_t off = l * n;
-			const uint64_t q = t->c.q;
-			switch (j.op) {
-			case VKHEL_JOB_NTT_FWD:
-				fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
-				break;
-			case VKHEL_JOB_NTT_INV:
-				inv.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, true));
-				break;
-			case VKHEL_JOB_POLYMUL: {
-				uint64_t *tmp = sp;
-				sp += n;
-				fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
-				fwd.push_back(make_task(*t, j.b->dev + off, tmp, false));
-				pt.push_back(PointTask{j.res->dev + off, tmp, j.res->dev + off, n, q,
-						barrett_mu(q), (uint32_t) (bitlen(q) - 2), 0});
-				inv.push_back(make_task(*t, j.res->dev + off, j.res->dev + off, true));
-				break;
-			}
-			case VKHEL_JOB_MUL:
-			case VKHEL_JOB_ADD:
-				pt.push_back(PointTask{j.a->dev + off, j.b->dev + off,
-						j.res->dev + off, n, q, barrett_mu(q),
-						(uint32_t) (bitlen(q) - 2),
-						(uint32_t) (j.op == VKHEL_JOB_MUL ? 0 : 1)});
-				break;
-			}
-			pt_maxlen = std::max<uint64_t>(pt_maxlen, n);
-		}
-	}
-	b->seg_fwd = segment(fwd);
-	b->seg_inv = segment(inv);
-	if (!upload(fwd, &b->d_fwd) || !upload(inv, &b->d_inv) ||
-			!upload(pt, &b->d_pt))
-		return vkhel_ext_last_error();
-	b->h_fwd = std::move(fwd);
-	b->h_inv = std::move(inv);
-	b->n_pt = (uint32_t) pt.size();
-	b->pt_maxlen = pt_maxlen;
+	std::vector<const Job *> jp;
+	for (const Job &j : b->jobs)
+		jp.push_back(&j);
+	int rc = plan_tasks(ctx, jp.data(), jp.size(), b->scratch, b->ts);
+	if (rc == VKHEL_OK)
+		rc = upload_tasks(b->ts, b->dt, ctx->stream);
+	if (rc == VKHEL_OK)
+		rc = hip_ok(hipStreamSynchronize(ctx->stream), "sync") ? VKHEL_OK
+			: VKHEL_ERR_DEVICE;
+	if (rc != VKHEL_OK)
+		return rc;
 
 	// Capture the launch sequence once; replays skip per-kernel launch cost.
 	hipStream_t s = ctx->stream;
 	if (!hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
 				"begin capture"))
 		return vkhel_ext_last_error();
-	const int rc = enqueue(b, s);
+	rc = enqueue_tasks(ctx, b->ts, b->dt, s);
 	hipGraph_t g = nullptr;
 	if (!hip_ok(hipStreamEndCapture(s, &g), "end capture") || rc != VKHEL_OK) {
 		if (g)
@@ -270,7 +337,8 @@ void vkhel_batch_destroy(struct vkhel_batch *b) {
 	if (!b)
 		return;
 	(void) hipStreamSynchronize(b->ctx->stream);
-	release_device(b);
+	release_graph(b);
+	b->dt.release();
 	(void) hipFree(b->scratch);
 	delete b;
 }
@@ -289,33 +357,12 @@ uint64_t vkhel_batch_num_jobs(const struct vkhel_batch *b) {
 int vkhel_batch_add(struct vkhel_batch *b, int op, const struct vkhel_vector *a,
 		const struct vkhel_vector *bv, struct vkhel_vector *res,
 		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs) {
-	if (!b || !a || !res || !tables || nlimbs == 0)
-		return fail(VKHEL_ERR_INVALID, "batch_add: null argument");
-	if (op < VKHEL_JOB_NTT_FWD || op > VKHEL_JOB_ADD)
-		return fail(VKHEL_ERR_INVALID, "batch_add: unknown op %d", op);
-	const bool binary = op >= VKHEL_JOB_POLYMUL;
-	if (binary && !bv)
-		return fail(VKHEL_ERR_INVALID, "batch_add: op needs a second operand");
-	if (a->ctx != b->ctx || res->ctx != b->ctx || (binary && bv->ctx != b->ctx))
-		return fail(VKHEL_ERR_INVALID, "batch_add: vector from another context");
-	for (uint32_t i = 0; i < nlimbs; i++)
-		if (!tables[i] || tables[i]->n != tables[0]->n)
-			return fail(VKHEL_ERR_INVALID,
-					"batch_add: limb tables must all exist and share n");
-	const uint64_t words = (uint64_t) nlimbs * tables[0]->n;
-	if (a->length < words || res->length < words ||
-			(binary && bv->length < words))
-		return fail(VKHEL_ERR_INVALID,
-				"batch_add: vectors need %llu words (%u limbs x n)",
-				(unsigned long long) words, nlimbs);
-	if (op == VKHEL_JOB_POLYMUL && (bv == res))
-		return fail(VKHEL_ERR_INVALID, "batch_add: polymul result aliases b");
+	if (!b)
+		return fail(VKHEL_ERR_INVALID, "batch_add: null batch");
 	Job j;
-	j.op = op;
-	j.a = a;
-	j.b = binary ? bv : nullptr;
-	j.res = res;
-	j.tables.assign(tables, tables + nlimbs);
+	const int rc = make_job(b->ctx, op, a, bv, res, tables, nlimbs, &j);
+	if (rc != VKHEL_OK)
+		return rc;
 	b->jobs.push_back(std::move(j));
 	b->dirty = true;
 	return VKHEL_OK;

@@ -98,6 +98,19 @@ __global__ __launch_bounds__(256) void k_unary(const uint64_t *a,
 		out[len - 1] = op(a[len - 1]);
 }
 
+// Synthetic residues: high half of splitmix64(seed, i) * q.
+__global__ __launch_bounds__(256) void k_fill_uniform(uint64_t *p, uint64_t n,
+		uint64_t q, uint64_t seed) {
+	for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
+			i += (uint64_t) gridDim.x * blockDim.x) {
+		uint64_t z = seed + (i + 1) * 0x9e3779b97f4a7c15ull;
+		z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+		z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+		z ^= z >> 31;
+		p[i] = mulhi64(z, q);
+	}
+}
+
 static unsigned grid_for(uint64_t len) {
 	const uint64_t blocks = ((len >> 1) + 255) / 256;
 	return (unsigned) std::max<uint64_t>(1, std::min<uint64_t>(blocks, 8192));
@@ -230,6 +243,21 @@ void vkhel_vector_elemgtsub(const struct vkhel_vector *a,
 		(uint32_t) (bitlen(mod) - 2)};
 	finish(res->ctx, launch_unary(a->dev, res->dev, res->length, op,
 				res->ctx->stream));
+}
+
+int vkhel_ext_vector_fill_uniform(struct vkhel_vector *v, uint64_t offset,
+		uint64_t count, uint64_t q, uint64_t seed) {
+	if (!v || offset > v->length || count > v->length - offset || q == 0)
+		return fail(VKHEL_ERR_INVALID, "fill_uniform: bad range or modulus");
+	if (count == 0)
+		return VKHEL_OK;
+	const unsigned grid = (unsigned) std::min<uint64_t>((count + 255) / 256, 8192);
+	hipLaunchKernelGGL(k_fill_uniform, dim3(grid), dim3(256), 0, v->ctx->stream,
+			v->dev + offset, count, q, seed);
+	if (!hip_ok(hipGetLastError(), "fill launch") ||
+			!hip_ok(hipStreamSynchronize(v->ctx->stream), "sync"))
+		return VKHEL_ERR_DEVICE;
+	return VKHEL_OK;
 }
 
 }  // extern "C"

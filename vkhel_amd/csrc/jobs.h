@@ -1,0 +1,68 @@
+// Job model shared by the batch API (batch.hip) and the serving engine
+// (serve.hip): a job is one operation on an RNS polynomial (nlimbs limbs of
+// n words, limb i modulo tables[i]->q); a set of jobs is planned into three
+// device phases of tasks (forward transforms, pointwise, inverse).
+#pragma once
+
+#include <vector>
+
+#include "internal.h"
+
+namespace vk {
+
+struct Job {
+	int op;
+	const vkhel_vector *a, *b;
+	vkhel_vector *res;
+	std::vector<const vkhel_ntt_tables *> tables;
+};
+
+// One pointwise limb: out = a (op) b over len words mod q.
+struct PointTask {
+	const uint64_t *a, *b;
+	uint64_t *out;
+	uint64_t len, q, mu;
+	uint32_t shift, op;  // op: 0 = mul (elemmul semantics), 1 = add
+};
+
+// A run of transform tasks with one log2 n.
+struct Segment {
+	uint32_t off, count, logn;
+};
+
+struct TaskSet {
+	std::vector<NttTask> fwd, inv;
+	std::vector<Segment> seg_fwd, seg_inv;
+	std::vector<PointTask> pt;
+	uint64_t pt_maxlen = 0;
+};
+
+// Device copies of a TaskSet's arrays, grown on demand.
+struct DeviceTasks {
+	NttTask *fwd = nullptr, *inv = nullptr;
+	PointTask *pt = nullptr;
+	size_t cap_fwd = 0, cap_inv = 0, cap_pt = 0;
+	void release();
+};
+
+// Checks a job's arguments and fills `out`.  Returns a vkhel_status.
+int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
+		const vkhel_vector *b, vkhel_vector *res,
+		vkhel_ntt_tables *const *tables, uint32_t nlimbs, Job *out);
+// Device scratch words a job needs (the spectrum of b for POLYMUL).
+uint64_t job_scratch_words(const Job &j);
+// Device bytes a job touches (operands, result, scratch): the serving
+// engine's admission cost.
+uint64_t job_bytes(const Job &j);
+// Transform coefficients a job processes (for throughput accounting).
+uint64_t job_transform_coeffs(const Job &j);
+// Plans `jobs` into tasks; POLYMUL scratch is carved from `scratch`.
+int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
+		uint64_t *scratch, TaskSet &ts);
+// Copies the task arrays to `dt` (stream-ordered) and enqueues the three
+// phases on `s`.
+int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s);
+int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
+		const DeviceTasks &dt, hipStream_t s);
+
+}  // namespace vk

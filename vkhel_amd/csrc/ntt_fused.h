@@ -358,7 +358,7 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 		k_ntt_pass<L + kMinFusedLogn, 2, true, TB>,
 		k_ntt_pass<L + kMinFusedLogn, 1, true, TB>,
 		FusedShape<L + kMinFusedLogn, TB>::THREADS,
-		L + kMinFusedLogn > TB ? 1 << (L + kMinFusedLogn - TB) : 1}...};
+		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0)}...};
 }
 
 // Per-tile-size tables (ntt_fused_kt*.hip).  Sizes n <= 2^TB use the

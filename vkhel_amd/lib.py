@@ -64,6 +64,15 @@ SIGNATURES = {
     "vkhel_ext_set_tile_bits": (_int, [_p, _u32, _int]),
     "vkhel_ext_autotune_save": (_int, [_p, ctypes.c_char_p]),
     "vkhel_ext_autotune_load": (_int, [_p, ctypes.c_char_p]),
+    "vkhel_server_create": (_p, [_p, _p]),
+    "vkhel_server_destroy": (None, [_p]),
+    "vkhel_server_submit": (_u64, [_p, _int, _p, _p, _p, ctypes.POINTER(_p), _u32]),
+    "vkhel_server_wait": (_int, [_p, _u64, _u64, _p]),
+    "vkhel_server_cancel": (_int, [_p, _u64]),
+    "vkhel_server_drain": (_int, [_p]),
+    "vkhel_server_get_stats": (_int, [_p, _p]),
+    "vkhel_server_replay": (_int, [_p, _p, _u64, ctypes.c_double, _u64, _p]),
+    "vkhel_ext_vector_fill_uniform": (_int, [_p, _u64, _u64, _u64, _u64]),
     "vkhel_batch_create": (_p, [_p]),
     "vkhel_batch_destroy": (None, [_p]),
     "vkhel_batch_clear": (None, [_p]),

@@ -164,6 +164,11 @@ class Vector:
         a = np.ascontiguousarray(arr, dtype=np.uint64)
         _lib.check(_L().vkhel_ext_vector_write(self._h, offset, a.size, _u64p(a)))
 
+    def fill_uniform(self, q, seed, offset=0, count=None):
+        """Synthetic residues in [0, q), generated on the device."""
+        count = self.length - offset if count is None else count
+        _lib.check(_L().vkhel_ext_vector_fill_uniform(self._h, offset, count, q, seed))
+
     def dup(self):
         h = _L().vkhel_vector_dup(self._h)
         if not h:
@@ -239,4 +244,112 @@ class Batch:
     def close(self):
         if self._h:
             _L().vkhel_batch_destroy(self._h)
+            self._h = None
+
+
+class ServerConfig(ctypes.Structure):
+    """struct vkhel_server_config (include/vkhel/vkhel_ext.h)."""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64),
+                ("max_inflight", ctypes.c_uint32),
+                ("max_batch", ctypes.c_uint32),
+                ("streams", ctypes.c_uint32),
+                ("queue_capacity", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    """struct vkhel_request_info."""
+    _fields_ = [("status", ctypes.c_int),
+                ("submit_ns", ctypes.c_uint64),
+                ("admit_ns", ctypes.c_uint64),
+                ("complete_ns", ctypes.c_uint64),
+                ("batch_size", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    """struct vkhel_server_stats."""
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("cancelled", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("transform_coeffs", ctypes.c_uint64),
+                ("inflight_bytes", ctypes.c_uint64),
+                ("peak_inflight_bytes", ctypes.c_uint64),
+                ("inflight", ctypes.c_uint32), ("waiting", ctypes.c_uint32)]
+
+
+OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_BUDGET, ERR_CANCELLED, ERR_TIMEOUT, \
+    ERR_BUSY = range(8)
+
+
+class JobDesc(ctypes.Structure):
+    """struct vkhel_job_desc."""
+    _fields_ = [("op", ctypes.c_int), ("a", ctypes.c_void_p), ("b", ctypes.c_void_p),
+                ("res", ctypes.c_void_p), ("tables", ctypes.c_void_p),
+                ("nlimbs", ctypes.c_uint32)]
+
+
+class Server:
+    """vkhel_server_*: continuous batching over device-resident operands."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=0, max_batch=0,
+                 streams=0, queue_capacity=0):
+        self.ctx = ctx
+        self.config = ServerConfig(hbm_budget_bytes, max_inflight, max_batch,
+                                   streams, queue_capacity)
+        self._h = _L().vkhel_server_create(ctx.handle, ctypes.byref(self.config))
+        if not self._h:
+            _lib.check()
+
+    def submit(self, op, a, b, res, tables):
+        """Returns the request id; raises VkhelError (BUDGET, BUSY, ...)."""
+        tabs = list(tables)
+        arr = (ctypes.c_void_p * len(tabs))(*[t.handle for t in tabs])
+        rid = _L().vkhel_server_submit(self._h, op, a.handle,
+                                       b.handle if b is not None else None,
+                                       res.handle, arr, len(tabs))
+        if not rid:
+            _lib.check()
+            raise _lib.VkhelError(ERR_INVALID, "submit failed")
+        return int(rid)
+
+    def wait(self, rid, timeout_ns=0):
+        """(status, info); status ERR_TIMEOUT leaves the request pending."""
+        info = RequestInfo()
+        st = _L().vkhel_server_wait(self._h, rid, timeout_ns, ctypes.byref(info))
+        if st == ERR_INVALID:
+            _lib.check()
+        return st, info
+
+    def cancel(self, rid):
+        _lib.check(_L().vkhel_server_cancel(self._h, rid))
+
+    def drain(self):
+        _lib.check(_L().vkhel_server_drain(self._h))
+
+    def replay(self, jobs, rate_per_s, seed=1):
+        """Submits (op, a, b, res, tables) jobs at Poisson arrivals, waits for
+        all; returns (status, [RequestInfo])."""
+        n = len(jobs)
+        descs = (JobDesc * n)()
+        keep = []
+        for d, (op, a, b, res, tables) in zip(descs, jobs):
+            arr = (ctypes.c_void_p * len(tables))(*[t.handle for t in tables])
+            keep.append(arr)
+            d.op, d.a = op, a.handle
+            d.b = b.handle if b is not None else None
+            d.res, d.tables = res.handle, ctypes.cast(arr, ctypes.c_void_p)
+            d.nlimbs = len(tables)
+        infos = (RequestInfo * n)()
+        st = _L().vkhel_server_replay(self._h, descs, n, float(rate_per_s), seed, infos)
+        if st == ERR_INVALID:
+            _lib.check()
+        _L().vkhel_ext_clear_error()
+        return st, list(infos)
+
+    def stats(self):
+        s = ServerStats()
+        _lib.check(_L().vkhel_server_get_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def close(self):
+        if self._h:
+            _L().vkhel_server_destroy(self._h)
             self._h = None
