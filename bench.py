@@ -1,22 +1,29 @@
 """Benchmark: batched negacyclic NTT throughput on one MI355X.
 
-Workload (the headline configuration): 8 RNS primes q_i < 2^50 with
-q_i = 1 mod 2n, 64 polynomials per prime (512 polynomials), n = 2^16 uint64
-coefficients uniform in [0, q_i).  One step = forward NTT of all 512
-polynomials then inverse NTT of all 512 (in place, so every step sees the
-same data), issued as one replayed hipGraph on the library's stream.
+Headline workload (`value`): 8 RNS primes q_i < 2^50 with q_i = 1 mod 2n,
+64 polynomials per prime (512 polynomials), n = 2^16 uint64 coefficients
+uniform in [0, q_i).  One step = forward NTT of all 512 polynomials then
+inverse NTT of all 512 (in place, so every step sees the same data), issued
+as one replayed hipGraph on the library's stream.  `value` is transform-
+coefficients per second (2 * 512 * n per step) with inputs resident in HBM.
 
-Reported `value` is transform-coefficients per second (2 * 512 * n per
-step) in Gcoeff/s with inputs resident in HBM.  Also reported:
-  * roofline of the dominant kernel (the fused forward NTT), timed with HIP
-    events on the library stream;
-  * the plain per-call path (vkhel_vector_forward_transform /
-    _inverse_transform, one synchronous call per polynomial) for the same
-    work;
-  * cpu_baseline: the oracle's scalar C NTT (same arithmetic) on the host
-    cores, over a bounded sample.
+Also reported, each with its own timing:
+  * roofline of the dominant kernel (fused forward NTT at n = 2^16), timed
+    with HIP events on the library stream;
+  * plain_per_call: the same transforms through the synchronous reference
+    API (vkhel_vector_forward_transform / _inverse_transform), one call per
+    polynomial;
+  * batched_n14: the same batch at n = 2^14;
+  * polymul: 256 pairs x 16 primes < 2^55 at n = 2^15 (NTT a, NTT b,
+    pointwise, INTT) as one batch, vs per call;
+  * uneven_batch: 1000 seeded jobs, n = 2^10..2^17, 1..24 primes < 2^60,
+    ops NTT / INTT / polymul / add / mul, as one batch and per call;
+  * serving: the same 1000 jobs through the serving engine at Poisson
+    arrival rates, p50/p99 latency beside a FIFO of plain per-call service;
+  * cpu_baseline: the oracle's scalar C NTT on the host cores (bounded
+    sample).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--logn 16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import ctypes
@@ -31,8 +38,6 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-
-M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 
 def splitmix64_stream(seed, count):
@@ -88,6 +93,7 @@ def is_prime(n):
 
 
 def primes_ntt(bound, m, count):
+    """Up to `count` primes q < bound with q = 1 mod m, descending."""
     out, c = [], ((bound - 2) // m) * m + 1
     while len(out) < count and c > m:
         if is_prime(c):
@@ -96,21 +102,34 @@ def primes_ntt(bound, m, count):
     return out
 
 
+_GEN = {}
+
+
+def smallest_generator(q):
+    if q not in _GEN:
+        try:
+            import sympy
+            _GEN[q] = int(sympy.primitive_root(q))
+        except ImportError:
+            m, fs, p = q - 1, [], 2
+            while p * p <= m:
+                if m % p == 0:
+                    fs.append(p)
+                    while m % p == 0:
+                        m //= p
+                p += 1 if p == 2 else 2
+            if m > 1:
+                fs.append(m)
+            g = 2
+            while any(pow(g, (q - 1) // f, q) == 1 for f in fs):
+                g += 1
+            _GEN[q] = g
+    return _GEN[q]
+
+
 def root_2n(q, n):
     """psi = g^((q-1)/2n), g the smallest generator of (Z/qZ)^*."""
-    m, fs, p = q - 1, [], 2
-    while p * p <= m:
-        if m % p == 0:
-            fs.append(p)
-            while m % p == 0:
-                m //= p
-        p += 1 if p == 2 else 2
-    if m > 1:
-        fs.append(m)
-    g = 2
-    while any(pow(g, (q - 1) // f, q) == 1 for f in fs):
-        g += 1
-    return pow(g, (q - 1) // (2 * n), q)
+    return pow(smallest_generator(q), (q - 1) // (2 * n), q)
 
 
 class HipEvents:
@@ -123,7 +142,6 @@ class HipEvents:
         self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float),
                                                ctypes.c_void_p, ctypes.c_void_p]
-        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
 
     def new(self):
         e = ctypes.c_void_p()
@@ -138,6 +156,273 @@ class HipEvents:
         out = ctypes.c_float()
         assert self.h.hipEventElapsedTime(ctypes.byref(out), a, b) == 0
         return float(out.value)
+
+
+def timed_launches(ctx, batch, reps, ev):
+    """Average ms of `reps` replays of `batch`, HIP events on ctx.stream."""
+    batch.run()
+    e0, e1 = ev.new(), ev.new()
+    ev.record(e0, ctx.stream)
+    for _ in range(reps):
+        batch.launch()
+    ev.record(e1, ctx.stream)
+    return ev.ms(e0, e1) / reps
+
+
+def pct(xs, p):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), p))
+
+
+# ---------------------------------------------------------------- legs
+
+def batched_ntt(V, ctx, logn, primes, per_prime, steps, warmup, ev):
+    """Config 2: forward + inverse of primes*per_prime polynomials."""
+    n = 1 << logn
+    qs = primes_ntt(1 << 50, 2 * n, primes)
+    tabs = [V.NttTables(n, q, root_2n(q, n)) for q in qs]
+    polys, data = [], []
+    for i in range(primes * per_prime):
+        v = ctx.vector(n, False)
+        v.fill_uniform(qs[i % primes], 1000 + i)
+        polys.append(v)
+    for i in (0, len(polys) - 1):
+        data.append((i, polys[i].read()))
+    both, fwd, inv = V.Batch(ctx), V.Batch(ctx), V.Batch(ctx)
+    for i, v in enumerate(polys):
+        both.add(V.JOB_NTT_FWD, v, None, v, [tabs[i % primes]])
+        fwd.add(V.JOB_NTT_FWD, v, None, v, [tabs[i % primes]])
+        inv.add(V.JOB_NTT_INV, v, None, v, [tabs[i % primes]])
+    for i, v in enumerate(polys):
+        both.add(V.JOB_NTT_INV, v, None, v, [tabs[i % primes]])
+    for _ in range(warmup):
+        both.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        both.launch()
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    for i, x in data:  # every step is NTT then INTT in place
+        assert np.array_equal(polys[i].read(), x), "round trip broken"
+    # dominant kernel: the fused forward NTT over all polynomials
+    reps = max(3, steps // 2)
+    fwd_ms = timed_launches(ctx, fwd, reps, ev)
+    inv_ms = timed_launches(ctx, inv, reps, ev)  # restores the data too
+    for _ in range(reps):
+        fwd.launch()
+    npoly = len(polys)
+    res = {"qs": qs, "tabs": tabs, "polys": polys, "n": n, "npoly": npoly,
+           "secs": secs, "steps": steps, "fwd_ms": fwd_ms, "inv_ms": inv_ms,
+           "gcoeff": 2 * npoly * n * steps / secs / 1e9,
+           "batches": (both, fwd, inv)}
+    return res
+
+
+def plain_per_call(V, ctx, r, k):
+    """The same transforms, one synchronous reference-API call each."""
+    tabs, polys, n = r["tabs"], r["polys"], r["n"]
+    primes = len(tabs)
+    k = min(k, len(polys))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k):
+        polys[i].forward_transform(polys[i], tabs[i % primes])
+        polys[i].inverse_transform(polys[i], tabs[i % primes])
+    secs = time.perf_counter() - t0
+    return {"value": round(2 * k * n / secs / 1e9, 4), "unit": "Gcoeff/s",
+            "polys": k, "us_per_call": round(secs / (2 * k) * 1e6, 2)}
+
+
+def polymul_leg(V, ctx, ev, pairs_per_prime=256, primes=16, logn=15, reps=5):
+    """Config 3: a*b mod (X^n + 1, q_i) for 256 pairs x 16 primes < 2^55."""
+    n = 1 << logn
+    qs = primes_ntt(1 << 55, 2 * n, primes)
+    tabs = [V.NttTables(n, q, root_2n(q, n)) for q in qs]
+    words = primes * n
+    A, B, R = [], [], []
+    for p in range(pairs_per_prime):
+        a, b, c = ctx.vector(words, False), ctx.vector(words, False), ctx.vector(words)
+        for i, q in enumerate(qs):
+            a.fill_uniform(q, 7 * p + i, i * n, n)
+            b.fill_uniform(q, 7 * p + 3 + i, i * n, n)
+        A.append(a)
+        B.append(b)
+        R.append(c)
+    bt = V.Batch(ctx)
+    for a, b, c in zip(A, B, R):
+        bt.add(V.JOB_POLYMUL, a, b, c, tabs)
+    ms = timed_launches(ctx, bt, reps, ev)
+    out_coeffs = pairs_per_prime * words
+    # per call on 8 pairs (limb views through the reference API)
+    sample = 8
+    tmp = ctx.vector(n)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for p in range(sample):
+        for i, (q, t) in enumerate(zip(qs, tabs)):
+            va, vb, vc = A[p].view(i * n, n), B[p].view(i * n, n), R[p].view(i * n, n)
+            va.forward_transform(vc, t)
+            vb.forward_transform(tmp, t)
+            vc.elemmul(tmp, vc, q)
+            vc.inverse_transform(vc, t)
+    plain_s = (time.perf_counter() - t0) / (sample * primes)
+    res = {"workload": f"{pairs_per_prime} pairs x {primes} primes <2^55, n=2^{logn}",
+           "ms": round(ms, 4),
+           "output_gcoeff_s": round(out_coeffs / ms / 1e6, 3),
+           "transform_gcoeff_s": round(3 * out_coeffs / ms / 1e6, 3),
+           "alg_GBps": round(24 * out_coeffs / ms / 1e6, 1),
+           "plain_per_call_output_gcoeff_s": round(n / plain_s / 1e9, 4)}
+    res["speedup_vs_plain"] = round(res["output_gcoeff_s"] /
+                                    res["plain_per_call_output_gcoeff_s"], 1)
+    for v in A + B + R + [tmp]:
+        v.close()
+    bt.close()
+    return res
+
+
+def make_uneven_jobs(V, ctx, njobs=1000, seed=20261202):
+    """Config 4: n = 2^10..2^17, L = 1..24 primes < 2^60 (q = 1 mod 2^18),
+    op uniform over NTT, INTT, polymul, add, mul."""
+    rng = np.random.default_rng(seed)
+    qs = primes_ntt(1 << 60, 1 << 18, 24)
+    tables = {}
+    jobs = []
+    for k in range(njobs):
+        logn = int(rng.integers(10, 18))
+        L = int(rng.integers(1, 25))
+        op = int(rng.integers(0, 5))
+        n = 1 << logn
+        for q in qs[:L]:
+            if (n, q) not in tables:
+                tables[(n, q)] = V.NttTables(n, q, root_2n(q, n))
+        tabs = [tables[(n, q)] for q in qs[:L]]
+        a = ctx.vector(L * n, False)
+        b = ctx.vector(L * n, False) if op >= V.JOB_POLYMUL else None
+        r = ctx.vector(L * n)
+        for i, q in enumerate(qs[:L]):
+            a.fill_uniform(q, (k << 8) + i, i * n, n)
+            if b is not None:
+                b.fill_uniform(q, (k << 8) + 128 + i, i * n, n)
+        jobs.append((op, a, b, r, tabs))
+    return jobs, tables
+
+
+def job_coeffs(V, job):
+    op, a, b, r, tabs = job
+    words = len(tabs) * tabs[0].n
+    return {V.JOB_NTT_FWD: words, V.JOB_NTT_INV: words,
+            V.JOB_POLYMUL: 3 * words}.get(op, 0)
+
+
+def per_call_service(V, ctx, jobs):
+    """Service time (s) of each job through synchronous reference-API calls
+    on limb views (NTT/INTT: 1 call per limb; polymul: 4; mul: elemmul;
+    add: elemfma with multiplier 1)."""
+    times = []
+    tmp = ctx.vector(1 << 17)
+    for op, a, b, r, tabs in jobs:
+        n = tabs[0].n
+        views = []
+        for i in range(len(tabs)):
+            views.append((a.view(i * n, n), b.view(i * n, n) if b is not None else None,
+                          r.view(i * n, n)))
+        tv = tmp.view(0, n)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for (va, vb, vr), t in zip(views, tabs):
+            if op == V.JOB_NTT_FWD:
+                va.forward_transform(vr, t)
+            elif op == V.JOB_NTT_INV:
+                va.inverse_transform(vr, t)
+            elif op == V.JOB_POLYMUL:
+                va.forward_transform(vr, t)
+                vb.forward_transform(tv, t)
+                vr.elemmul(tv, vr, t.q)
+                vr.inverse_transform(vr, t)
+            elif op == V.JOB_MUL:
+                va.elemmul(vb, vr, t.q)
+            else:
+                va.elemfma(vb, vr, 1, t.q)
+        times.append(time.perf_counter() - t0)
+    tmp.close()
+    return times
+
+
+def fifo_latencies(arrivals, service):
+    """Latency of each request for one synchronous server taking requests in
+    arrival order (completion_i = max(arrival_i, completion_{i-1}) + s_i)."""
+    done, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        done = max(a, done) + s
+        out.append(done - a)
+    return out
+
+
+def poisson_arrivals(njobs, rate, seed):
+    """Arrival times matching vkhel_server_replay's exponential gaps."""
+    st = seed
+    t, out = 0.0, []
+    M = (1 << 64) - 1
+    for _ in range(njobs):
+        st = (st + 0x9E3779B97F4A7C15) & M
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        u = ((z >> 11) + 1.0) / 9007199254740992.0
+        t += -np.log(u) / rate
+        out.append(t)
+    return out
+
+
+def uneven_and_serving(V, ctx, ev, quick=False):
+    jobs, tables = make_uneven_jobs(V, ctx, 200 if quick else 1000)
+    coeffs = sum(job_coeffs(V, j) for j in jobs)
+    bt = V.Batch(ctx)
+    for j in jobs:
+        bt.add(*j)
+    batch_ms = timed_launches(ctx, bt, 3, ev)
+    bt.close()
+    service = per_call_service(V, ctx, jobs)
+    plain_total = sum(service)
+    uneven = {"workload": f"{len(jobs)} jobs, n=2^10..2^17, L=1..24 primes <2^60, "
+                          "ops NTT/INTT/polymul/add/mul (seeded)",
+              "transform_coeffs": coeffs,
+              "batch_ms": round(batch_ms, 3),
+              "batched_gcoeff_s": round(coeffs / batch_ms / 1e6, 3),
+              "plain_per_call_gcoeff_s": round(coeffs / plain_total / 1e9, 4),
+              "plain_per_call_ms": round(plain_total * 1e3, 2),
+              "speedup_vs_plain": round(plain_total * 1e3 / batch_ms, 1)}
+    # serving: Poisson arrivals at fractions of the batched job rate
+    capacity = len(jobs) / (batch_ms * 1e-3)
+    points = []
+    srv = V.Server(ctx, streams=4)
+    for frac in ([0.5] if quick else [0.1, 0.25, 0.5, 0.75]):
+        rate = capacity * frac
+        st, infos = srv.replay(jobs, rate_per_s=rate, seed=77)
+        assert st == 0, st
+        lat = [(i.complete_ns - i.submit_ns) / 1e6 for i in infos]
+        span = (max(i.complete_ns for i in infos) - min(i.submit_ns for i in infos)) / 1e9
+        fifo = fifo_latencies(poisson_arrivals(len(jobs), rate, 77), service)
+        points.append({"rate_jobs_s": round(rate, 1),
+                       "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
+                       "gcoeff_s": round(coeffs / span / 1e9, 3),
+                       "mean_batch": round(float(np.mean([i.batch_size for i in infos])), 2),
+                       "plain_fifo_p50_ms": round(pct(fifo, 50) * 1e3, 3),
+                       "plain_fifo_p99_ms": round(pct(fifo, 99) * 1e3, 3)})
+    st = srv.stats()
+    srv.close()
+    serving = {"streams": 4, "hbm_budget_bytes": 200 * 10**9,
+               "batched_capacity_jobs_s": round(capacity, 1),
+               "plain_service_p50_ms": round(pct(service, 50) * 1e3, 3),
+               "plain_service_p99_ms": round(pct(service, 99) * 1e3, 3),
+               "points": points, "peak_inflight_bytes": int(st.peak_inflight_bytes)}
+    for op, a, b, r, tabs in jobs:
+        a.close()
+        r.close()
+        if b is not None:
+            b.close()
+    return uneven, serving
 
 
 def cpu_baseline(logn, qs, ws, target_cpu_s=10.0):
@@ -156,7 +441,33 @@ def cpu_baseline(logn, qs, ws, target_cpu_s=10.0):
             "cores": threads, "kind": "port",
             "sample": f"{npolys} polynomials of n=2^{logn} ({len(qs)} primes <2^50), "
                       f"forward+inverse, scalar C (oracle/vkhel_oracle.c) "
-                      f"over {threads} threads, {secs:.2f} s wall"}
+                      f"over {threads} threads, {secs:.2f} s wall",
+            "one_core_gcoeff_s": round(2 * 4 * n / t1 / 1e9, 5)}
+
+
+def config1(V, ctx):
+    """Config 1: one N=4096 polymul, q the largest 60-bit prime = 1 mod
+    8192; CPU oracle (one thread) vs the per-call GPU API."""
+    from oracle import oracle as O
+    n = 4096
+    q = primes_ntt(1 << 60, 2 * n, 1)[0]
+    w = root_2n(q, n)
+    a, b = uniform_mod(q, n, 1), uniform_mod(q, n, 2)
+    t0 = time.perf_counter()
+    want = O.polymul_ntt(a, b, q, w)
+    cpu_s = time.perf_counter() - t0
+    t = V.NttTables(n, q, w)
+    va, vb, vc = ctx.from_array(a), ctx.from_array(b), ctx.vector(n)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    va.forward_transform(va, t)
+    vb.forward_transform(vb, t)
+    va.elemmul(vb, vc, q)
+    vc.inverse_transform(vc, t)
+    gpu_s = time.perf_counter() - t0
+    ok = bool(np.array_equal(vc.read(), want))
+    return {"q": q, "psi": w, "cpu_oracle_ms": round(cpu_s * 1e3, 3),
+            "gpu_per_call_ms": round(gpu_s * 1e3, 3), "bit_exact": ok}
 
 
 def main():
@@ -167,9 +478,10 @@ def main():
     ap.add_argument("--logn", type=int, default=16)
     ap.add_argument("--primes", type=int, default=8)
     ap.add_argument("--per-prime", type=int, default=64)
-    ap.add_argument("--plain-polys", type=int, default=64,
-                    help="polynomials timed on the plain per-call path")
+    ap.add_argument("--plain-polys", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the secondary legs (polymul, uneven, serving)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; "
@@ -178,100 +490,72 @@ def main():
 
     import vkhel_amd as V
 
-    n = 1 << args.logn
-    qs = primes_ntt(1 << 50, 2 * n, args.primes)
-    ws = [root_2n(q, n) for q in qs]
-    ctx = V.Context()
-    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
-    polys, data = [], []
-    for i in range(args.primes * args.per_prime):
-        x = uniform_mod(qs[i % args.primes], n, 1000 + i)
-        data.append(x)
-        polys.append(ctx.from_array(x))
-    both, fwd = V.Batch(ctx), V.Batch(ctx)
-    for i, v in enumerate(polys):
-        both.add(V.JOB_NTT_FWD, v, None, v, [tabs[i % args.primes]])
-    for i, v in enumerate(polys):
-        both.add(V.JOB_NTT_INV, v, None, v, [tabs[i % args.primes]])
-    for i, v in enumerate(polys):
-        fwd.add(V.JOB_NTT_FWD, v, None, v, [tabs[i % args.primes]])
-
-    for _ in range(args.warmup):
-        both.run()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        both.launch()
-    ctx.synchronize()
-    secs = time.perf_counter() - t0
-    npoly = len(polys)
-    coeffs_per_step = 2 * npoly * n
-    gcoeff = coeffs_per_step * args.steps / secs / 1e9
-
-    # Results of the timed steps: every step is NTT then INTT in place.
-    for i in (0, npoly - 1):
-        assert np.array_equal(polys[i].read(), data[i]), "round trip broken"
-
-    # Dominant kernel: fused forward NTT over all 512 polynomials.
     ev = HipEvents()
-    stream = ctx.stream
-    reps = max(3, args.steps // 2)
-    e0, e1 = ev.new(), ev.new()
-    fwd.run()  # build + warm
-    ev.record(e0, stream)
-    for _ in range(reps):
-        fwd.launch()
-    ev.record(e1, stream)
-    fwd_ms = ev.ms(e0, e1) / reps
-    # restore data (fwd-only runs changed it); not timed
-    inv = V.Batch(ctx)
-    for i, v in enumerate(polys):
-        inv.add(V.JOB_NTT_INV, v, None, v, [tabs[i % args.primes]])
-    for _ in range(reps + 1):
-        inv.run()
-    alg_bytes = 16 * npoly * n + 16 * n * args.primes
-    achieved = alg_bytes / (fwd_ms * 1e-3) / 1e9
+    ctx = V.Context()
+    t_tune = time.perf_counter()
+    ctx.autotune(12, 17)
+    tuned = {l: ctx.tile_bits(l) for l in range(12, 18)}
+    t_tune = time.perf_counter() - t_tune
 
-    # Plain per-call path: one synchronous call per polynomial.
-    k = min(args.plain_polys, npoly)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for i in range(k):
-        polys[i].forward_transform(polys[i], tabs[i % args.primes])
-        polys[i].inverse_transform(polys[i], tabs[i % args.primes])
-    plain_secs = time.perf_counter() - t0
-    plain = 2 * k * n / plain_secs / 1e9
+    r = batched_ntt(V, ctx, args.logn, args.primes, args.per_prime, args.steps,
+                    args.warmup, ev)
+    n, npoly = r["n"], r["npoly"]
+    alg_bytes = 16 * npoly * n + 16 * n * args.primes
+    achieved = alg_bytes / (r["fwd_ms"] * 1e-3) / 1e9
+    plain = plain_per_call(V, ctx, r, args.plain_polys)
+    plain["speedup"] = round(r["gcoeff"] / plain["value"], 1)
+    ws = [t.w for t in r["tabs"]]
+    qs = r["qs"]
+    for v in r["polys"]:
+        v.close()
+    for b in r["batches"]:
+        b.close()
+
+    extra = {}
+    if not args.headline_only:
+        r14 = batched_ntt(V, ctx, 14, 8, 64, args.steps, args.warmup, ev)
+        extra["batched_n14"] = {"value": round(r14["gcoeff"], 3), "unit": "Gcoeff/s",
+                                "fwd_kernel_ms": round(r14["fwd_ms"], 4),
+                                "alg_GBps": round(16 * r14["npoly"] * r14["n"] /
+                                                  (r14["fwd_ms"] * 1e-3) / 1e9, 1)}
+        for v in r14["polys"]:
+            v.close()
+        for b in r14["batches"]:
+            b.close()
+        extra["polymul"] = polymul_leg(V, ctx, ev)
+        extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
+        extra["config1"] = config1(V, ctx)
 
     cpu = None if args.no_cpu else cpu_baseline(args.logn, qs, ws)
-
     out = {
         "metric": "batched NTT Gcoeff/s & %HBM peak at N=2^16; served p50/p99 vs plain per-call",
-        "value": round(gcoeff, 3),
+        "value": round(r["gcoeff"], 3),
         "unit": "Gcoeff/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(secs / args.steps * 1e3, 4),
+        "ms_per_step": round(r["secs"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic (splitmix64, uniform in [0,q))",
+        "data": "synthetic (device splitmix64, uniform in [0,q))",
         "config": {"workload": f"batched NTT fwd+inv, n=2^{args.logn}, "
                                f"{args.primes} primes <2^50 x {args.per_prime} polys",
                    "n": n, "polys": npoly, "primes": qs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_ntt_fused<fwd>", "kernel_ms": round(fwd_ms, 4),
+                     "kernel": "k_ntt_fused<16, fwd>", "kernel_ms": round(r["fwd_ms"], 4),
+                     "inverse_kernel_ms": round(r["inv_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes},
-        "plain_per_call": {"value": round(plain, 4), "unit": "Gcoeff/s",
-                           "polys": k, "speedup": round(gcoeff / plain, 2)},
+        "plain_per_call": plain,
+        "tile_bits": tuned,
+        "autotune_s": round(t_tune, 2),
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
-    for v in polys:
-        v.close()
 
 
 if __name__ == "__main__":

@@ -48,6 +48,11 @@ int vkhel_ext_ctx_synchronize(struct vkhel_ctx *);
 const char *vkhel_ext_ctx_device_name(struct vkhel_ctx *);
 uint64_t *vkhel_ext_vector_device_ptr(struct vkhel_vector *);
 uint64_t vkhel_ext_vector_length(const struct vkhel_vector *);
+/* A vector aliasing words [offset, offset + length) of `base` (e.g. one
+ * RNS limb), usable wherever a vector is; destroying it frees only the
+ * handle.  It must not outlive `base`. */
+struct vkhel_vector *vkhel_ext_vector_view(struct vkhel_vector *base,
+		uint64_t offset, uint64_t length);
 /* Synchronous device<->host copies of `count` words starting at `offset`. */
 int vkhel_ext_vector_read(const struct vkhel_vector *, uint64_t offset,
 		uint64_t count, uint64_t *host);

@@ -82,6 +82,7 @@ struct vkhel_vector {
 	uint64_t *dev;
 	void *staging;  // pinned host buffer while mapped
 	size_t staging_words;
+	bool view;      // aliases another vector's memory (vkhel_ext_vector_view)
 };
 
 namespace vk {

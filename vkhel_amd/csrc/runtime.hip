@@ -283,7 +283,8 @@ void vkhel_vector_destroy(struct vkhel_vector *v) {
 	(void) hipStreamSynchronize(v->ctx->stream);
 	if (v->staging)
 		(void) hipHostFree(v->staging);
-	(void) hipFree(v->dev);
+	if (!v->view)
+		(void) hipFree(v->dev);
 	free(v);
 }
 
@@ -349,6 +350,21 @@ void vkhel_vector_unmap(struct vkhel_vector *v) {
 	(void) hipHostFree(v->staging);
 	v->staging = nullptr;
 	v->staging_words = 0;
+}
+
+struct vkhel_vector *vkhel_ext_vector_view(struct vkhel_vector *base,
+		uint64_t offset, uint64_t length) {
+	if (!base || offset > base->length || length > base->length - offset)
+		return fail(VKHEL_ERR_INVALID, "vector_view: range out of bounds"),
+			nullptr;
+	vkhel_vector *v = (vkhel_vector *) calloc(1, sizeof *v);
+	if (!v)
+		return fail(VKHEL_ERR_NOMEM, "vector_view: host allocation"), nullptr;
+	v->ctx = base->ctx;
+	v->length = length;
+	v->dev = base->dev + offset;
+	v->view = true;
+	return v;
 }
 
 void *vkhel_ext_ctx_stream(struct vkhel_ctx *ctx) {

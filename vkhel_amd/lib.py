@@ -53,6 +53,7 @@ SIGNATURES = {
     "vkhel_ext_ctx_device_name": (ctypes.c_char_p, [_p]),
     "vkhel_ext_vector_device_ptr": (_p, [_p]),
     "vkhel_ext_vector_length": (_u64, [_p]),
+    "vkhel_ext_vector_view": (_p, [_p, _u64, _u64]),
     "vkhel_ext_vector_read": (_int, [_p, _u64, _u64, _pu64]),
     "vkhel_ext_vector_write": (_int, [_p, _u64, _u64, _pu64]),
     "vkhel_ext_tables_info": (_int, [_p, _pu64, _pu64, _pu64]),

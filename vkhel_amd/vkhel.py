@@ -169,6 +169,15 @@ class Vector:
         count = self.length - offset if count is None else count
         _lib.check(_L().vkhel_ext_vector_fill_uniform(self._h, offset, count, q, seed))
 
+    def view(self, offset, length):
+        """vkhel_ext_vector_view: a vector aliasing [offset, offset+length)."""
+        h = _L().vkhel_ext_vector_view(self._h, offset, length)
+        if not h:
+            _lib.check()
+        v = Vector(self.ctx, 0, _handle=h)
+        v._base = self  # keep the aliased vector alive
+        return v
+
     def dup(self):
         h = _L().vkhel_vector_dup(self._h)
         if not h:
