@@ -208,8 +208,6 @@ def batched_ntt(V, ctx, logn, primes, per_prime, steps, warmup, ev):
     reps = max(3, steps // 2)
     fwd_ms = timed_launches(ctx, fwd, reps, ev)
     inv_ms = timed_launches(ctx, inv, reps, ev)  # restores the data too
-    for _ in range(reps):
-        fwd.launch()
     npoly = len(polys)
     res = {"qs": qs, "tabs": tabs, "polys": polys, "n": n, "npoly": npoly,
            "secs": secs, "steps": steps, "fwd_ms": fwd_ms, "inv_ms": inv_ms,
@@ -470,6 +468,18 @@ def config1(V, ctx):
             "gpu_per_call_ms": round(gpu_s * 1e3, 3), "bit_exact": ok}
 
 
+def traffic_from_profile(logn, npoly):
+    """HBM bytes per launch of the forward kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary for this workload (scaled by
+    the copy-kernel calibration recorded beside it), or None."""
+    path = os.path.join(REPO, "profiles", f"ntt_fused_fwd_n{1 << logn}_x{npoly}.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -482,6 +492,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the secondary legs (polymul, uneven, serving)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 copies of 256 MiB (elemgtadd with bound "
+                         "2^64-1) to calibrate FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; "
@@ -492,6 +505,15 @@ def main():
 
     ev = HipEvents()
     ctx = V.Context()
+    if args.copy_calibration:
+        words = 32 << 20
+        a, b = ctx.vector(words, False), ctx.vector(words, False)
+        a.fill_uniform(1 << 62, 1)
+        for _ in range(3):
+            a.elemgtadd(b, (1 << 64) - 1, 0)  # x > 2^64-1 never holds: a copy
+        print(json.dumps({"copy_bytes_read": words * 8, "copy_bytes_written": words * 8,
+                          "launches": 3}))
+        return
     t_tune = time.perf_counter()
     ctx.autotune(12, 17)
     tuned = {l: ctx.tile_bits(l) for l in range(12, 18)}
@@ -545,7 +567,8 @@ def main():
                    "n": n, "polys": npoly, "primes": qs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(args.logn, npoly),
                      "kernel": "k_ntt_fused<16, fwd>", "kernel_ms": round(r["fwd_ms"], 4),
                      "inverse_kernel_ms": round(r["inv_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes},
