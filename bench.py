@@ -423,8 +423,10 @@ def uneven_and_serving(V, ctx, ev, quick=False):
     return uneven, serving
 
 
-def cpu_baseline(logn, qs, ws, target_cpu_s=10.0):
-    """Oracle scalar NTT (fwd+inv) on host cores over a bounded sample."""
+def cpu_baseline(logn, qs, ws, target_cpu_s=16.0):
+    """Oracle scalar NTT (fwd+inv) on host cores over a bounded sample:
+    passes over up to 4096 resident polynomials until about target_cpu_s
+    thread-seconds of work are done."""
     from oracle import oracle as O
     n = 1 << logn
     threads = max(1, min(os.cpu_count() or 1, 16))
@@ -434,12 +436,16 @@ def cpu_baseline(logn, qs, ws, target_cpu_s=10.0):
     npolys = int(min(max(threads, target_cpu_s / per_poly), 4096))
     npolys = max(len(qs), npolys - npolys % len(qs))
     polys = np.stack([uniform_mod(qs[i % len(qs)], n, 100 + i) for i in range(npolys)])
-    secs = O.bench_ntt_batch(polys, n, qs, ws, threads)
-    return {"value": round(2 * npolys * n / secs / 1e9, 4), "unit": "Gcoeff/s",
-            "cores": threads, "kind": "port",
-            "sample": f"{npolys} polynomials of n=2^{logn} ({len(qs)} primes <2^50), "
-                      f"forward+inverse, scalar C (oracle/vkhel_oracle.c) "
-                      f"over {threads} threads, {secs:.2f} s wall",
+    secs, passes = 0.0, 0
+    while passes < 16 and (passes == 0 or secs * threads < target_cpu_s):
+        secs += O.bench_ntt_batch(polys, n, qs, ws, threads)
+        passes += 1
+    return {"value": round(2 * passes * npolys * n / secs / 1e9, 4),
+            "unit": "Gcoeff/s", "cores": threads, "kind": "port",
+            "sample": f"{passes} passes over {npolys} polynomials of n=2^{logn} "
+                      f"({len(qs)} primes <2^50), forward+inverse, scalar C "
+                      f"(oracle/vkhel_oracle.c) over {threads} threads, "
+                      f"{secs:.2f} s wall",
             "one_core_gcoeff_s": round(2 * 4 * n / t1 / 1e9, 5)}
 
 

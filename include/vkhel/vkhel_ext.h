@@ -84,16 +84,22 @@ int vkhel_ext_set_ntt_algo(struct vkhel_ctx *, int algo);
 /* ---- autotuning of the fused NTT kernels ----
  * Sizes above 2^11 run as two fused passes over tiles of 2^11, 2^12 or
  * 2^13 words (the tile size fixes the stage split and workgroup shape).
- * vkhel_ext_autotune times each compiled tile size for every log2 n in
- * [logn_lo, logn_hi] (3..20) on `batch` synthetic polynomials (0: enough
- * to fill the device) and keeps the fastest per size on this context. */
+ * vkhel_ext_autotune times each compiled tile size, with both pass modes
+ * (below), for every log2 n in [logn_lo, logn_hi] (3..20) on `batch`
+ * synthetic polynomials (0: enough to fill the device) and keeps the
+ * fastest per size on this context. */
 int vkhel_ext_autotune(struct vkhel_ctx *, uint32_t logn_lo, uint32_t logn_hi,
 		uint32_t batch);
 /* Tile size (log2 words) in use for log2 n, or -1 on error. */
 int vkhel_ext_tile_bits(struct vkhel_ctx *, uint32_t logn);
 /* tile_bits 11..13, or 0 for the built-in default. */
 int vkhel_ext_set_tile_bits(struct vkhel_ctx *, uint32_t logn, int tile_bits);
-/* Tuned choices as text lines "log2n tile_bits". */
+/* Pass mode for log2 n: 0 = built-in default (split), 1 = one workgroup
+ * per polynomial for both passes, 2 = one grid per pass with one workgroup
+ * per tile (split).  The autotuner sets it too. */
+int vkhel_ext_pass_mode(struct vkhel_ctx *, uint32_t logn);
+int vkhel_ext_set_pass_mode(struct vkhel_ctx *, uint32_t logn, int mode);
+/* Tuned choices as text lines "log2n tile_bits pass_mode". */
 int vkhel_ext_autotune_save(struct vkhel_ctx *, const char *path);
 int vkhel_ext_autotune_load(struct vkhel_ctx *, const char *path);
 

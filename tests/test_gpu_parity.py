@@ -314,16 +314,21 @@ def test_batch_roundtrip_full_size(ctx):
 
 # ---------------- tile sizes, split vs whole-polynomial kernels ----------------
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("tb", [11, 12, 13])
 @pytest.mark.parametrize("logn", [12, 13, 14, 16, 17, 18])
-def test_tile_sizes_per_call_and_batched(ctx, logn, tb):
+def test_tile_sizes_per_call_and_batched(ctx, logn, tb, mode):
+    """Every compiled tile size in both pass modes: 1 = one workgroup per
+    polynomial running both passes, 2 = one grid per pass, one workgroup per
+    tile."""
     n = 1 << logn
     q = _prime(logn, 55)
     w = O.root_2n(q, n)
     t = V.NttTables(n, q, w)
     ctx.set_tile_bits(logn, tb)
+    ctx.set_pass_mode(logn, mode)
     try:
-        # per-call path: one polynomial, split passes (one workgroup per tile)
+        # per-call path: one polynomial
         x = O.fill_uniform(n, q, seed=logn + tb)
         a = vec(ctx, x)
         f = ctx.vector(n)
@@ -332,7 +337,7 @@ def test_tile_sizes_per_call_and_batched(ctx, logn, tb):
         assert np.array_equal(f.read(), want)
         f.inverse_transform(f, t)
         assert np.array_equal(f.read(), x)
-        # large batch: one workgroup per polynomial, two fused passes
+        # batch of 80 polynomials through the task array
         npoly = 80
         rng = np.random.default_rng(logn * 100 + tb)
         xs = [rng.integers(0, q, n, dtype=np.uint64) for _ in range(npoly)]
@@ -353,18 +358,22 @@ def test_tile_sizes_per_call_and_batched(ctx, logn, tb):
         ib.close()
     finally:
         ctx.set_tile_bits(logn, 0)
+        ctx.set_pass_mode(logn, 0)
         t.close()
 
 
 def test_autotune_choices_and_cache(ctx, tmp_path):
     ctx.autotune(12, 14, 64)
-    chosen = {l: ctx.tile_bits(l) for l in (12, 13, 14)}
-    assert all(tb in (11, 12, 13) for tb in chosen.values()), chosen
+    chosen = {l: (ctx.tile_bits(l), ctx.pass_mode(l)) for l in (12, 13, 14)}
+    assert all(tb in (11, 12, 13) and m in (1, 2)
+               for tb, m in chosen.values()), chosen
     path = tmp_path / "tune.txt"
     ctx.autotune_save(path)
     for l in chosen:
         ctx.set_tile_bits(l, 0)
+        ctx.set_pass_mode(l, 0)
     ctx.autotune_load(path)
-    assert {l: ctx.tile_bits(l) for l in chosen} == chosen
+    assert {l: (ctx.tile_bits(l), ctx.pass_mode(l)) for l in chosen} == chosen
     for l in chosen:
         ctx.set_tile_bits(l, 0)
+        ctx.set_pass_mode(l, 0)

@@ -50,6 +50,19 @@ int vkhel_ext_set_tile_bits(struct vkhel_ctx *ctx, uint32_t logn,
 	return VKHEL_OK;
 }
 
+int vkhel_ext_pass_mode(struct vkhel_ctx *ctx, uint32_t logn) {
+	if (!ctx || logn >= 64)
+		return fail(VKHEL_ERR_INVALID, "pass_mode: bad argument"), -1;
+	return ctx->pass_mode[logn];
+}
+
+int vkhel_ext_set_pass_mode(struct vkhel_ctx *ctx, uint32_t logn, int mode) {
+	if (!ctx || logn >= 64 || mode < 0 || mode > 2)
+		return fail(VKHEL_ERR_INVALID, "set_pass_mode: mode must be 0, 1 or 2");
+	ctx->pass_mode[logn] = mode;
+	return VKHEL_OK;
+}
+
 int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 		uint32_t logn_hi, uint32_t batch) {
 	if (!ctx || logn_lo < 3 || logn_hi > 20 || logn_lo > logn_hi)
@@ -104,12 +117,16 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 		t.inv = tw;
 		t.c = c;
 		const int saved = ctx->tile_bits[logn];
+		const int saved_mode = ctx->pass_mode[logn];
 		float best = 1e30f;
-		int best_tb = kDefaultTileBits;
-		for (int tb : kTileChoices) {
-			if (logn <= (uint32_t) tb && tb != 13)
+		int best_tb = tile_bits_for(ctx, logn), best_mode = 0;
+		for (int choice = 0; choice < 6; choice++) {
+			const int tb = kTileChoices[choice % 3];
+			const int mode = 1 + choice / 3;  // whole polynomial, then split
+			if (logn <= (uint32_t) tb && (tb != 13 || mode == 2))
 				continue;  // one-tile size: every table holds the same kernel
 			ctx->tile_bits[logn] = tb;
+			ctx->pass_mode[logn] = mode;
 			int rc = VKHEL_OK;
 			for (int rep = 0; rep < 2 && rc == VKHEL_OK; rep++) {  // warm-up
 				rc = launch_ntt(ctx, t, data, data, polys, false, s);
@@ -128,6 +145,7 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 			if (rc != VKHEL_OK || !hip_ok(hipEventSynchronize(e1), "sync") ||
 					!hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed")) {
 				ctx->tile_bits[logn] = saved;
+				ctx->pass_mode[logn] = saved_mode;
 				(void) hipFree(data);
 				(void) hipFree(tw);
 				return vkhel_ext_last_error();
@@ -135,9 +153,11 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 			if (ms < best) {
 				best = ms;
 				best_tb = tb;
+				best_mode = mode;
 			}
 		}
 		ctx->tile_bits[logn] = best_tb;
+		ctx->pass_mode[logn] = best_mode;
 		(void) hipEventDestroy(e0);
 		(void) hipEventDestroy(e1);
 		(void) hipFree(data);
@@ -146,7 +166,7 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 	return VKHEL_OK;
 }
 
-// Text format: one "log2n tile_bits" pair per line.
+// Text format: one "log2n tile_bits pass_mode" triple per line.
 int vkhel_ext_autotune_save(struct vkhel_ctx *ctx, const char *path) {
 	if (!ctx || !path)
 		return fail(VKHEL_ERR_INVALID, "autotune_save: null argument");
@@ -154,8 +174,8 @@ int vkhel_ext_autotune_save(struct vkhel_ctx *ctx, const char *path) {
 	if (!f)
 		return fail(VKHEL_ERR_INVALID, "autotune_save: cannot open %s", path);
 	for (uint32_t l = 0; l < ctx->tile_bits.size(); l++)
-		if (ctx->tile_bits[l])
-			fprintf(f, "%u %d\n", l, ctx->tile_bits[l]);
+		if (ctx->tile_bits[l] || ctx->pass_mode[l])
+			fprintf(f, "%u %d %d\n", l, ctx->tile_bits[l], ctx->pass_mode[l]);
 	fclose(f);
 	return VKHEL_OK;
 }
@@ -167,14 +187,17 @@ int vkhel_ext_autotune_load(struct vkhel_ctx *ctx, const char *path) {
 	if (!f)
 		return fail(VKHEL_ERR_INVALID, "autotune_load: cannot open %s", path);
 	unsigned l;
-	int tb;
+	int tb, mode;
 	int rc = VKHEL_OK;
-	while (fscanf(f, "%u %d", &l, &tb) == 2) {
-		if (l >= ctx->tile_bits.size() || tb < 11 || tb > 13) {
-			rc = fail(VKHEL_ERR_INVALID, "autotune_load: bad entry %u %d", l, tb);
+	while (fscanf(f, "%u %d %d", &l, &tb, &mode) == 3) {
+		if (l >= ctx->tile_bits.size() || (tb != 0 && (tb < 11 || tb > 13)) ||
+				mode < 0 || mode > 2) {
+			rc = fail(VKHEL_ERR_INVALID, "autotune_load: bad entry %u %d %d", l, tb,
+					mode);
 			break;
 		}
 		ctx->tile_bits[l] = tb;
+		ctx->pass_mode[l] = mode;
 	}
 	fclose(f);
 	return rc;

@@ -72,6 +72,9 @@ struct vkhel_ctx {
 	int ntt_algo = VKHEL_NTT_AUTO;
 	// Tuned tile size (log2 words) per log2 n; 0 = default.
 	std::vector<int> tile_bits = std::vector<int>(64, 0);
+	// Tuned pass mode per log2 n: 0 = by batch size, 1 = whole polynomial
+	// per workgroup, 2 = one grid per pass (one workgroup per tile).
+	std::vector<int> pass_mode = std::vector<int>(64, 0);
 	std::mutex mu;  // guards `tables` (batch and serve threads share it)
 	std::unordered_map<uint64_t, vk::DevTables *> tables;
 };
@@ -100,10 +103,6 @@ struct NttTask {
 	const ulonglong2 *tw;
 	NttConsts c;
 };
-
-// Tile size (log2 words) of the fused two-pass kernels when no tuned value
-// exists for a size.
-constexpr int kDefaultTileBits = 13;
 
 // Launchers (ntt.hip).  `batch` polynomials of n words, poly p at
 // in + p*n / out + p*n, all with the same tables.  Stream-ordered.

@@ -18,6 +18,38 @@ __device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) {
 	return x >= m ? x - m : x;
 }
 
+// Makes a value opaque to the optimiser (emits no instruction), so the
+// arithmetic around it is kept as written instead of being re-matched into
+// compare/select or borrow chains.
+__device__ __forceinline__ uint64_t opaque(uint64_t v) {
+	asm("" : "+v"(v));
+	return v;
+}
+
+__device__ __forceinline__ uint32_t opaque32(uint32_t v) {
+	asm("" : "+v"(v));
+	return v;
+}
+
+// Same for a wave-uniform value kept in scalar registers.
+__device__ __forceinline__ uint64_t opaque_uniform(uint64_t v) {
+	asm("" : "+s"(v));
+	return v;
+}
+
+// csub for x, m < 2^63 without a compare, given nm = 2^64 - m: d = x - m
+// wraps to >= 2^63 exactly when x < m, and the sign word then adds m back.
+// Two 64-bit adds, a shift and two ANDs; nothing writes a scalar register
+// (a compare or a borrow would, and costs wait states before its reader).
+__device__ __forceinline__ uint64_t csub_lt63(uint64_t x, uint64_t m,
+		uint64_t nm) {
+	const uint64_t d = x + nm;
+	const uint32_t mask = opaque32((uint32_t) ((int32_t) (d >> 32) >> 31));
+	const uint64_t back = ((uint64_t) ((uint32_t) (m >> 32) & mask) << 32) |
+		((uint32_t) m & mask);
+	return d + back;
+}
+
 // (a - b) mod q for a, b in [0, q).
 __device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b,
 		uint64_t q) {

@@ -161,9 +161,18 @@ bool ntt_fused_supported(uint32_t logn) {
 	return fused_supported(logn);
 }
 
+// Untuned default tile per size, from batched timings on MI355X (2^25 words
+// per batch, split passes): 2^11 tiles up to n = 2^15, 2^12 at 2^16, 2^13
+// above; sizes up to 2^13 run in one pass on a 2^13 tile.
+static int default_tile_bits(uint32_t logn) {
+	if (logn <= 13)
+		return 13;
+	return logn <= 15 ? 11 : logn == 16 ? 12 : 13;
+}
+
 int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn) {
 	const int tb = logn < ctx->tile_bits.size() ? ctx->tile_bits[logn] : 0;
-	return tb ? tb : kDefaultTileBits;
+	return tb ? tb : default_tile_bits(logn);
 }
 
 // Launches `ntasks` transforms of one size and direction: either through
@@ -190,8 +199,13 @@ static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
 
 // Batches smaller than this many polynomials use the split passes so that
 // more than one workgroup works on each polynomial.
-static uint32_t split_threshold(const vkhel_ctx *ctx) {
-	return (uint32_t) std::max(1, ctx->cu_count / 4);
+// Two-pass sizes run their passes as separate grids (one workgroup per tile)
+// unless tuned otherwise: at every size and batch measured, the split grids'
+// smaller register footprint and full-chip occupancy beat one workgroup per
+// polynomial.
+static bool use_split(const vkhel_ctx *ctx, uint32_t logn) {
+	const int mode = logn < ctx->pass_mode.size() ? ctx->pass_mode[logn] : 0;
+	return mode != 1;
 }
 
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
@@ -202,7 +216,7 @@ int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
 		return fail(VKHEL_ERR_INVALID, "fused NTT: unsupported log2 n = %u", logn);
 	const NttTask none{};
 	return launch_fused_impl(none, dev_tasks, ntasks, logn,
-			tile_bits_for(ctx, logn), ntasks < split_threshold(ctx), inverse, st);
+			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, st);
 }
 
 int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
@@ -213,7 +227,7 @@ int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
 	if (ctx->ntt_algo == VKHEL_NTT_RADIX2 || !fused_supported(logn))
 		return launch_radix2(t, in, out, batch, inverse, s);
 	return launch_fused_impl(make_task(t, in, out, inverse), nullptr, batch, logn,
-			tile_bits_for(ctx, logn), batch < split_threshold(ctx), inverse, s);
+			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, s);
 }
 
 static void transform(const vkhel_vector *op, vkhel_vector *res,

@@ -33,20 +33,32 @@ constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
 // Inverse (Gentleman-Sande): x, y in [0, 2q) -> [0, 2q).  Both stay
 // congruent mod q to the reference's canonical butterflies
 // (nttfwdbutterfly.comp:44-57, nttrevbutterfly.comp:44-57).
+// Values stay below 4q < 2^63 (q < 2^61), so csub_lt63 applies, and
+// a + 2q - b is formed as (a + 2q + 1) + ~b: additions only, no borrow.
+// Constants of the lazy butterflies: 2q, 2^64 - 2q, 2q + 1, 2^64 - q.
+struct LazyK {
+	uint64_t two_q, n_two_q, two_q1, nq;
+};
+
+__device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
+	return LazyK{c.two_q, opaque_uniform(0 - c.two_q),
+		opaque_uniform(c.two_q + 1), opaque_uniform(0 - c.q)};
+}
+
 __device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
-		uint64_t two_q, uint64_t nq) {
-	const uint64_t xr = csub(x, two_q);
-	const uint64_t t = shoup32(y, w.x, w.y, nq);
+		const LazyK &k) {
+	const uint64_t xr = csub_lt63(x, k.two_q, k.n_two_q);
+	const uint64_t t = shoup32(y, w.x, w.y, k.nq);
 	x = xr + t;
-	y = xr + two_q - t;
+	y = (xr + k.two_q1) + opaque(~t);
 }
 
 __device__ __forceinline__ void bf_inv(uint64_t &x, uint64_t &y, ulonglong2 w,
-		uint64_t two_q, uint64_t nq) {
-	const uint64_t s = csub(x + y, two_q);
-	const uint64_t d = x + two_q - y;
+		const LazyK &k) {
+	const uint64_t s = csub_lt63(x + y, k.two_q, k.n_two_q);
+	const uint64_t d = (x + k.two_q1) + opaque(~y);
 	x = s;
-	y = shoup32(d, w.x, w.y, nq);
+	y = shoup32(d, w.x, w.y, k.nq);
 }
 
 // First forward stage on arbitrary 64-bit words, exactly as the reference:
@@ -120,6 +132,28 @@ struct Round {
 	static constexpr bool DIRECT = RUN >= 4;
 };
 
+// Global-memory accesses through address-space-1 pointers, so they compile
+// to global_load/store (task pointers are generic; flat accesses would also
+// hold the LDS wait counter).
+typedef __attribute__((address_space(1))) uint64_t gu64;
+typedef __attribute__((address_space(1))) const uint64_t gcu64;
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const u64x2 gcu128;
+
+__device__ __forceinline__ uint64_t gload(const uint64_t *p, uint32_t i) {
+	return ((gcu64 *) p)[i];
+}
+
+__device__ __forceinline__ void gstore(uint64_t *p, uint32_t i, uint64_t v) {
+	((gu64 *) p)[i] = v;
+}
+
+__device__ __forceinline__ ulonglong2 gload_tw(const ulonglong2 *p,
+		uint32_t i) {
+	const u64x2 v = ((gcu128 *) p)[i];
+	return make_ulonglong2(v.x, v.y);
+}
+
 // One padding word per 8 keeps the radix-8 access patterns conflict free.
 __device__ __forceinline__ uint32_t pad(uint32_t l) {
 	return l + (l >> 3);
@@ -164,7 +198,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 #pragma unroll
 			for (int k = 0; k < 8; k++) {
 				const uint32_t l = t + k * G::THREADS;
-				lds[pad(l)] = src[tile_addr<G>(l, jbase)];
+				lds[pad(l)] = gload(src, tile_addr<G>(l, jbase));
 			}
 		}
 		__syncthreads();
@@ -175,7 +209,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 		for (int e = 0; e < 8; e++) {
 			const uint32_t l = l0 + (e << F);
 			if (FIRST_ROUND && RD::DIRECT)
-				v[e] = src[tile_addr<G>(l, jbase)];
+				v[e] = gload(src, tile_addr<G>(l, jbase));
 			else
 				v[e] = lds[pad(l)];
 		}
@@ -189,13 +223,13 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 			const uint32_t ti = (1u << (LOGN - 1 - b)) + ((gbase + l0) >> (lb + 1));
 #pragma unroll
 			for (int gi = 0; gi < (4 >> d); gi++)
-				w[tw_slot(d, gi)] = tw[ti + gi];
+				w[tw_slot(d, gi)] = gload_tw(tw, ti + gi);
 		}
 	}
 
 	// ---- butterflies ----
 	if (active) {
-		const uint64_t nq = 0 - c.q;
+		const LazyK lk = lazy_consts(c);
 #pragma unroll
 		for (int k = 0; k < 3; k++) {
 			const int d = INV ? k : 2 - k;
@@ -218,11 +252,9 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 					for (int li = 0; li < (1 << d); li++) {
 						const int e = (gi << (d + 1)) | li;
 						if (INV)
-							bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
-									c.two_q, nq);
+							bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 						else
-							bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
-									c.two_q, nq);
+							bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 					}
 				}
 			}
@@ -230,7 +262,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 		if (!INV && LAST_ROUND && final) {
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				v[e] = csub(csub(v[e], c.two_q), c.q);
+				v[e] = csub_lt63(csub_lt63(v[e], lk.two_q, lk.n_two_q), c.q, lk.nq);
 		}
 	}
 
@@ -239,7 +271,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 		if (active) {
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				dst[tile_addr<G>(l0 + (e << F), jbase)] = v[e];
+				gstore(dst, tile_addr<G>(l0 + (e << F), jbase), v[e]);
 		}
 		__syncthreads();
 	} else {
@@ -254,7 +286,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 #pragma unroll
 				for (int k = 0; k < 8; k++) {
 					const uint32_t l = t + k * G::THREADS;
-					dst[tile_addr<G>(l, jbase)] = lds[pad(l)];
+					gstore(dst, tile_addr<G>(l, jbase), lds[pad(l)]);
 				}
 			}
 			__syncthreads();

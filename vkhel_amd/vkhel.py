@@ -64,6 +64,14 @@ class Context:
     def set_tile_bits(self, logn, tile_bits):
         _lib.check(_L().vkhel_ext_set_tile_bits(self._h, logn, tile_bits))
 
+    def pass_mode(self, logn):
+        m = _L().vkhel_ext_pass_mode(self._h, logn)
+        _lib.check()
+        return m
+
+    def set_pass_mode(self, logn, mode):
+        _lib.check(_L().vkhel_ext_set_pass_mode(self._h, logn, mode))
+
     def autotune_save(self, path):
         _lib.check(_L().vkhel_ext_autotune_save(self._h, str(path).encode()))
 
