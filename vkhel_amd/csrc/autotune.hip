@@ -89,7 +89,7 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 		std::vector<ulonglong2> htw(n);
 		for (uint64_t i = 0; i < n; i++) {
 			const uint64_t w = (0x9e3779b97f4a7c15ull * (i + 1)) % q;
-			htw[i] = make_ulonglong2(w, barrett_factor(w, q));
+			htw[i] = make_ulonglong2(w, shoup63_factor(w, q));
 		}
 		NttConsts c{};
 		c.q = q;
@@ -99,7 +99,7 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 		c.bshift = (uint32_t) (bitlen(q) - 2);
 		c.logn = logn;
 		c.ninv = invmod_euclid(n, q);
-		c.ninv_p = barrett_factor(c.ninv, q);
+		c.ninv_p = shoup63_factor(c.ninv, q);
 		c.w1n = c.ninv;
 		c.w1n_p = c.ninv_p;
 		hipEvent_t e0, e1;

@@ -31,7 +31,7 @@ struct NttConsts {
 	uint64_t r64;     // 2^64 mod q: correction for the wrapped X + WY sum
 	uint64_t mu;      // floor(2^(bits+62) / q), bits = bitlen(q)
 	uint64_t ninv;    // n^-1 mod q
-	uint64_t ninv_p;  // Shoup companion of ninv
+	uint64_t ninv_p;  // floor(ninv * 2^63 / q), the 63-bit Shoup companion
 	uint64_t w1n;     // inv_roots[1] * n^-1 mod q (last inverse stage)
 	uint64_t w1n_p;
 	uint32_t bshift;  // bits - 2

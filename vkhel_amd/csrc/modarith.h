@@ -13,14 +13,8 @@ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
 	return __umul64hi(a, b);
 }
 
-// x - m if x >= m else x.
-__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) {
-	return x >= m ? x - m : x;
-}
-
 // Makes a value opaque to the optimiser (emits no instruction), so the
-// arithmetic around it is kept as written instead of being re-matched into
-// compare/select or borrow chains.
+// arithmetic around it is kept as written instead of being reassociated.
 __device__ __forceinline__ uint64_t opaque(uint64_t v) {
 	asm("" : "+v"(v));
 	return v;
@@ -31,23 +25,11 @@ __device__ __forceinline__ uint32_t opaque32(uint32_t v) {
 	return v;
 }
 
-// Same for a wave-uniform value kept in scalar registers.
-__device__ __forceinline__ uint64_t opaque_uniform(uint64_t v) {
-	asm("" : "+s"(v));
-	return v;
-}
-
-// csub for x, m < 2^63 without a compare, given nm = 2^64 - m: d = x - m
-// wraps to >= 2^63 exactly when x < m, and the sign word then adds m back.
-// Two 64-bit adds, a shift and two ANDs; nothing writes a scalar register
-// (a compare or a borrow would, and costs wait states before its reader).
-__device__ __forceinline__ uint64_t csub_lt63(uint64_t x, uint64_t m,
-		uint64_t nm) {
-	const uint64_t d = x + nm;
-	const uint32_t mask = opaque32((uint32_t) ((int32_t) (d >> 32) >> 31));
-	const uint64_t back = ((uint64_t) ((uint32_t) (m >> 32) & mask) << 32) |
-		((uint32_t) m & mask);
-	return d + back;
+// x - m if x >= m else x: one 64-bit subtraction whose borrow selects the
+// result (sub, sub-with-borrow, two selects).
+__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) {
+	uint64_t d;
+	return __builtin_sub_overflow(x, m, &d) ? x : d;
 }
 
 // (a - b) mod q for a, b in [0, q).
@@ -69,25 +51,29 @@ __device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w,
 	return csub(shoup_lazy(y, w, wp, q), q);
 }
 
-// Shoup product on 32-bit halves, result in [0, 2q) like shoup_lazy:
-//   Q = floor(y * wp / 2^64) exactly, R = y*w + Q*(2^64 - q) mod 2^64.
-// nq = 2^64 - q.  Written out so every partial product is one
-// v_mad_u64_u32 / v_mul_hi_u32 / v_mul_lo_u32 and the final subtraction
-// folds into multiply-adds (no borrow chain).
-__device__ __forceinline__ uint64_t shoup32(uint64_t y, uint64_t w,
+// Shoup product with a 63-bit companion wp = floor(w * 2^63 / q), on
+// 32-bit halves.  For y < 2^63 and w < q < 2^61 it returns
+// R = y*w - Q*q, Q = floor(y * wp / 2^63), and R lies in [0, 2q):
+// y*w/q - Q < 1 + y*(w*2^63/q - wp)/2^63 < 2.
+// Because y1 = y >> 32 and p1 = wp >> 32 are both below 2^31, the middle
+// partial-product sum S = y0*p1 + y1*p0 + floor(y0*p0 / 2^32) stays below
+// 2^64, so Q = 2*y1*p1 + floor(S / 2^31) exactly, in four multiply-adds and
+// one shift, with no carry chain.  R is formed modulo 2^64 as
+// y*w + Q*nq with nq = 2^64 - q.
+__device__ __forceinline__ uint64_t shoup63(uint64_t y, uint64_t w,
 		uint64_t wp, uint64_t nq) {
 	const uint32_t y0 = (uint32_t) y, y1 = (uint32_t) (y >> 32);
 	const uint32_t p0 = (uint32_t) wp, p1 = (uint32_t) (wp >> 32);
-	const uint64_t a = __umulhi(y0, p0);
-	const uint64_t b = (uint64_t) y0 * p1 + a;             // < 2^64
-	const uint64_t m = (uint64_t) y1 * p0 + (uint32_t) b;  // < 2^64
-	const uint64_t qh = (uint64_t) y1 * p1 + (b >> 32) + (m >> 32);
+	// (the opaque() points keep each sum a single multiply-add)
+	const uint64_t s0 = opaque((uint64_t) y0 * p1 + __umulhi(y0, p0));
+	const uint64_t s = (uint64_t) y1 * p0 + s0;
+	const uint64_t qt = (uint64_t) y1 * (opaque32(p1) << 1) + (s >> 31);
 	const uint32_t w0 = (uint32_t) w, w1 = (uint32_t) (w >> 32);
 	const uint32_t n0 = (uint32_t) nq, n1 = (uint32_t) (nq >> 32);
-	const uint32_t q0 = (uint32_t) qh, q1 = (uint32_t) (qh >> 32);
+	const uint32_t q0 = (uint32_t) qt, q1 = (uint32_t) (qt >> 32);
 	const uint64_t r = (uint64_t) y0 * w0 + (uint64_t) q0 * n0;
-	const uint32_t rh = (uint32_t) (r >> 32) + y0 * w1 + y1 * w0 + q0 * n1 +
-		q1 * n0;
+	const uint32_t rh = opaque32((uint32_t) (r >> 32) + y0 * w1 + y1 * w0 +
+			q0 * n1 + q1 * n0);
 	return ((uint64_t) rh << 32) | (uint32_t) r;
 }
 

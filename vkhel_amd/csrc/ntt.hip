@@ -46,7 +46,9 @@ __global__ __launch_bounds__(256) void k_fwd_stage(const uint64_t *in,
 		const uint64_t y = x + ((uint64_t) 1 << hb);
 		const ulonglong2 w = tw[((uint64_t) 1 << s) + grp];
 		const uint64_t X = in[x], Y = in[y];
-		const uint64_t wy = shoup(Y, w.x, w.y, c.q);
+		// twiddle pairs hold 63-bit Shoup companions (inputs below 2^63)
+		const uint64_t wy = csub(shoup63(FIRST ? barrett64(Y, c.q, c.mu,
+						c.bshift) : Y, w.x, w.y, 0 - c.q), c.q);
 		uint64_t ox, oy;
 		if (FIRST) {
 			const uint64_t xr = barrett64(X, c.q, c.mu, c.bshift);
@@ -81,12 +83,12 @@ __global__ __launch_bounds__(256) void k_inv_stage(const uint64_t *in,
 		const uint64_t X = in[x], Y = in[y];
 		const uint64_t sum = csub(X + Y, c.q), d = sub_mod(X, Y, c.q);
 		if (LAST) {
-			out[x] = shoup(sum, c.ninv, c.ninv_p, c.q);
-			out[y] = shoup(d, c.w1n, c.w1n_p, c.q);
+			out[x] = csub(shoup63(sum, c.ninv, c.ninv_p, 0 - c.q), c.q);
+			out[y] = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
 		} else {
 			const ulonglong2 w = tw[((uint64_t) 1 << s) + grp];
 			out[x] = sum;
-			out[y] = shoup(d, w.x, w.y, c.q);
+			out[y] = csub(shoup63(d, w.x, w.y, 0 - c.q), c.q);
 		}
 	}
 }

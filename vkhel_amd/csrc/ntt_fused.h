@@ -32,41 +32,52 @@ constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
 // Lazy butterflies.  Forward (Cooley-Tukey): x, y in [0, 4q) -> [0, 4q).
 // Inverse (Gentleman-Sande): x, y in [0, 2q) -> [0, 2q).  Both stay
 // congruent mod q to the reference's canonical butterflies
-// (nttfwdbutterfly.comp:44-57, nttrevbutterfly.comp:44-57).
-// Values stay below 4q < 2^63 (q < 2^61), so csub_lt63 applies, and
-// a + 2q - b is formed as (a + 2q + 1) + ~b: additions only, no borrow.
-// Constants of the lazy butterflies: 2q, 2^64 - 2q, 2q + 1, 2^64 - q.
+// (nttfwdbutterfly.comp:44-57, nttrevbutterfly.comp:44-57).  Values stay
+// below 4q < 2^63 (q < 2^61), as shoup63 requires.  Twiddle pairs hold
+// {w, floor(w * 2^63 / q)}.
 struct LazyK {
-	uint64_t two_q, n_two_q, two_q1, nq;
+	uint64_t two_q, nq;  // 2q, 2^64 - q
 };
 
 __device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
-	return LazyK{c.two_q, opaque_uniform(0 - c.two_q),
-		opaque_uniform(c.two_q + 1), opaque_uniform(0 - c.q)};
+	return LazyK{c.two_q, 0 - c.q};
 }
 
 __device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
 		const LazyK &k) {
-	const uint64_t xr = csub_lt63(x, k.two_q, k.n_two_q);
-	const uint64_t t = shoup32(y, w.x, w.y, k.nq);
+	const uint64_t xr = csub(x, k.two_q);
+	const uint64_t t = shoup63(y, w.x, w.y, k.nq);
 	x = xr + t;
-	y = (xr + k.two_q1) + opaque(~t);
+	y = xr + k.two_q - t;
 }
 
 __device__ __forceinline__ void bf_inv(uint64_t &x, uint64_t &y, ulonglong2 w,
 		const LazyK &k) {
-	const uint64_t s = csub_lt63(x + y, k.two_q, k.n_two_q);
-	const uint64_t d = (x + k.two_q1) + opaque(~y);
+	const uint64_t s = csub(x + y, k.two_q);
+	const uint64_t d = x + k.two_q - y;
 	x = s;
-	y = shoup32(d, w.x, w.y, k.nq);
+	y = shoup63(d, w.x, w.y, k.nq);
+}
+
+// First forward stage when both inputs are already below q: the lazy
+// butterfly without its input reduction.  The reference's first stage
+// gives the canonical forms of these same residues.
+__device__ __forceinline__ void bf_fwd_first_canon(uint64_t &x, uint64_t &y,
+		ulonglong2 w, const LazyK &k) {
+	const uint64_t t = shoup63(y, w.x, w.y, k.nq);
+	const uint64_t xr = x;
+	x = xr + t;
+	y = xr + k.two_q - t;
 }
 
 // First forward stage on arbitrary 64-bit words, exactly as the reference:
-// (X + WY) mod 2^64 mod q and (X - WY) mod q with WY canonical.
+// (X + WY) mod 2^64 mod q and (X - WY) mod q with WY = W*Y mod q canonical
+// (nttfwdbutterfly.comp:44-57 on unreduced input words).
 __device__ __forceinline__ void bf_fwd_first(uint64_t &x, uint64_t &y,
 		ulonglong2 w, const NttConsts &c) {
 	const uint64_t X = x;
-	const uint64_t wy = shoup(y, w.x, w.y, c.q);
+	const uint64_t yr = barrett64(y, c.q, c.mu, c.bshift);
+	const uint64_t wy = csub(shoup63(yr, w.x, w.y, 0 - c.q), c.q);
 	const uint64_t xr = barrett64(X, c.q, c.mu, c.bshift);
 	uint64_t ox = csub(xr + wy, c.q);
 	if (X + wy < X)
@@ -76,13 +87,14 @@ __device__ __forceinline__ void bf_fwd_first(uint64_t &x, uint64_t &y,
 }
 
 // Last inverse stage (m = 1) with the reference's final multiplication by
-// n^-1 (src/vector.c:633-638) folded in; canonical outputs.
+// n^-1 (src/vector.c:633-638) folded in; canonical outputs.  ninv_p and
+// w1n_p are 63-bit Shoup companions.
 __device__ __forceinline__ void bf_inv_last(uint64_t &x, uint64_t &y,
 		const NttConsts &c) {
 	const uint64_t s = csub(x + y, c.two_q);
 	const uint64_t d = x + c.two_q - y;
-	x = shoup(s, c.ninv, c.ninv_p, c.q);
-	y = shoup(d, c.w1n, c.w1n_p, c.q);
+	x = csub(shoup63(s, c.ninv, c.ninv_p, 0 - c.q), c.q);
+	y = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
 }
 
 // Tiles hold 2^KT words of one polynomial, 8 per thread.  A pass runs a
@@ -180,10 +192,31 @@ __host__ __device__ constexpr int tw_slot(int d, int gi) {
 	return (d == 2 ? 0 : d == 1 ? 1 : 3) + gi;
 }
 
+// Loads every twiddle pair round R uses into w (slots as tw_slot).
 template <class G, bool INV, int R, int LOGN>
-__device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
-		const uint64_t *src, uint64_t *dst, uint64_t *lds, uint32_t jbase,
-		uint32_t gbase, const ulonglong2 *tw, const NttConsts &c, bool final) {
+__device__ __forceinline__ void load_round_tw(ulonglong2 w[7], uint32_t t,
+		uint32_t gbase, const ulonglong2 *tw) {
+	using RD = Round<G, INV, R>;
+	constexpr int F = RD::F;
+	const uint32_t l0 = round_base<F>(t);
+#pragma unroll
+	for (int d = 2; d >= 0; d--) {
+		const int lb = F + d;
+		if (lb < RD::BLO || lb >= RD::BHI)
+			continue;
+		const int b = lb + G::SHIFT;
+		const uint32_t ti = (1u << (LOGN - 1 - b)) + ((gbase + l0) >> (lb + 1));
+#pragma unroll
+		for (int gi = 0; gi < (4 >> d); gi++)
+			w[tw_slot(d, gi)] = gload_tw(tw, ti + gi);
+	}
+}
+
+template <class G, bool INV, int R, int LOGN>
+__device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
+		uint32_t t, const uint64_t *src, uint64_t *dst, uint64_t *lds,
+		uint32_t jbase, uint32_t gbase, const ulonglong2 *tw, const NttConsts &c,
+		bool final) {
 	using RD = Round<G, INV, R>;
 	constexpr int F = RD::F;
 	constexpr bool FIRST_ROUND = R == 0;
@@ -203,7 +236,6 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 		}
 		__syncthreads();
 	}
-	ulonglong2 w[7];
 	if (active) {
 #pragma unroll
 		for (int e = 0; e < 8; e++) {
@@ -213,18 +245,10 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 			else
 				v[e] = lds[pad(l)];
 		}
-		// every twiddle of the round, issued before any arithmetic
-#pragma unroll
-		for (int d = 2; d >= 0; d--) {
-			const int lb = F + d;
-			if (lb < RD::BLO || lb >= RD::BHI)
-				continue;
-			const int b = lb + G::SHIFT;
-			const uint32_t ti = (1u << (LOGN - 1 - b)) + ((gbase + l0) >> (lb + 1));
-#pragma unroll
-			for (int gi = 0; gi < (4 >> d); gi++)
-				w[tw_slot(d, gi)] = gload_tw(tw, ti + gi);
-		}
+		// the first round's twiddles; later rounds' were issued before the
+		// previous round's barrier
+		if (FIRST_ROUND)
+			load_round_tw<G, INV, R, LOGN>(w, t, gbase, tw);
 	}
 
 	// ---- butterflies ----
@@ -238,9 +262,22 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 				continue;
 			const int b = lb + G::SHIFT;
 			if (!INV && b == LOGN - 1) {
+				// Inputs below q (the usual case) take the lazy butterfly;
+				// a wave holding any larger word follows the reference's
+				// exact reduction.
+				bool canon = true;
 #pragma unroll
-				for (int e = 0; e < 4; e++)
-					bf_fwd_first(v[e], v[e + 4], w[0], c);
+				for (int e = 0; e < 8; e++)
+					canon &= v[e] < c.q;
+				if (__all(canon)) {
+#pragma unroll
+					for (int e = 0; e < 4; e++)
+						bf_fwd_first_canon(v[e], v[e + 4], w[0], lk);
+				} else {
+#pragma unroll
+					for (int e = 0; e < 4; e++)
+						bf_fwd_first(v[e], v[e + 4], w[0], c);
+				}
 			} else if (INV && b == LOGN - 1) {
 #pragma unroll
 				for (int e = 0; e < 4; e++)
@@ -262,7 +299,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 		if (!INV && LAST_ROUND && final) {
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				v[e] = csub_lt63(csub_lt63(v[e], lk.two_q, lk.n_two_q), c.q, lk.nq);
+				v[e] = csub(csub(v[e], lk.two_q), c.q);
 		}
 	}
 
@@ -279,6 +316,9 @@ __device__ __forceinline__ void run_round(uint64_t v[8], uint32_t t,
 #pragma unroll
 			for (int e = 0; e < 8; e++)
 				lds[pad(l0 + (e << F))] = v[e];
+			// next round's twiddles, in flight across the barrier
+			if constexpr (!LAST_ROUND)
+				load_round_tw<G, INV, R + 1, LOGN>(w, t, gbase, tw);
 		}
 		__syncthreads();
 		if (LAST_ROUND) {
@@ -300,7 +340,8 @@ __device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...>,
 		uint32_t jbase, uint32_t gbase, const ulonglong2 *tw, const NttConsts &c,
 		bool final) {
 	uint64_t v[8];
-	(run_round<G, INV, R, LOGN>(v, t, src, dst, lds, jbase, gbase, tw, c,
+	ulonglong2 w[7];
+	(run_round<G, INV, R, LOGN>(v, w, t, src, dst, lds, jbase, gbase, tw, c,
 			final), ...);
 }
 
@@ -355,8 +396,14 @@ __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 // One pass of a two-pass size, one workgroup per (tile, polynomial):
 // blockIdx.x = tile, blockIdx.y = polynomial.  The first pass of a
 // direction reads `in`, the second reads and writes `out`.
+#ifndef VKHEL_PASS_WAVES
+#define VKHEL_PASS_WAVES 8
+#endif
+// Eight waves per SIMD (at most 64 VGPRs): the passes are latency bound at
+// lower occupancy.
 template <int LOGN, int PASS, bool INV, int TB>
-__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_pass(
+__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
+__attribute__((amdgpu_waves_per_eu(VKHEL_PASS_WAVES))) void k_ntt_pass(
 		NttTask uniform, const NttTask *tasks, uint32_t second) {
 	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
 	if constexpr (LOGN > TB) {

@@ -20,6 +20,11 @@ inline uint64_t barrett_factor(uint64_t f, uint64_t q) {
 	return (uint64_t) (((u128) f << 64) / q);
 }
 
+// floor(f * 2^63 / q): the 63-bit Shoup companion the device kernels use.
+inline uint64_t shoup63_factor(uint64_t f, uint64_t q) {
+	return barrett_factor(f, q) >> 1;
+}
+
 // Barrett constant floor(2^(bits+62) / q) of elemmul.c:159-162 and
 // elemgtsub.c:154-157 (the factor argument 2^(bits-2) shifted by 64).
 inline uint64_t barrett_mu(uint64_t q) {

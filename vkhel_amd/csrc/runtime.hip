@@ -64,13 +64,13 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 		return it->second;
 
 	const uint64_t n = t->n, q = t->q;
-	// {w, w'} pairs for both directions.
+	// {w, floor(w * 2^63 / q)} pairs for both directions.
 	std::vector<ulonglong2> host(2 * n);
 	for (uint64_t i = 0; i < n; i++) {
 		host[i] = make_ulonglong2(t->roots_of_unity[i],
-				t->roots_barrett_factors[i]);
+				t->roots_barrett_factors[i] >> 1);
 		host[n + i] = make_ulonglong2(t->inv_roots_of_unity[i],
-				t->inv_roots_barrett_factors[i]);
+				t->inv_roots_barrett_factors[i] >> 1);
 	}
 	DevTables *d = new DevTables;
 	d->n = n;
@@ -95,9 +95,9 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 	c.logn = (uint32_t) (bitlen(n) - 1);
 	// src/vector.c:633: n^-1 by the reference's ext-Euclid.
 	c.ninv = invmod_euclid(n % q, q);
-	c.ninv_p = barrett_factor(c.ninv, q);
+	c.ninv_p = shoup63_factor(c.ninv, q);
 	c.w1n = mulmod_exact(t->inv_roots_of_unity[n > 1 ? 1 : 0], c.ninv, q);
-	c.w1n_p = barrett_factor(c.w1n, q);
+	c.w1n_p = shoup63_factor(c.w1n, q);
 	ctx->tables.emplace(t->id, d);
 	return d;
 }
