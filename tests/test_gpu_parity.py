@@ -165,7 +165,9 @@ def test_ntt_vs_oracle(ctx, logn, algo):
     n = 1 << logn
     ctx.set_ntt_algo(algo)
     try:
-        for bits in (50, 60):
+        # 61 bits: the largest moduli accepted, where the lazy bound 4q
+        # comes closest to 2^63
+        for bits in (50, 60, 61):
             q = _prime(logn, bits)
             w = O.root_2n(q, n)
             t = V.NttTables(n, q, w)
@@ -186,12 +188,13 @@ def test_ntt_vs_oracle(ctx, logn, algo):
         ctx.set_ntt_algo(V.NTT_AUTO)
 
 
+@pytest.mark.parametrize("bits", [50, 61])
 @pytest.mark.parametrize("logn", [1, 4, 10, 13, 14, 16])
-def test_ntt_noncanonical_inputs_match_reference(ctx, logn):
+def test_ntt_noncanonical_inputs_match_reference(ctx, logn, bits):
     # nttfwdbutterfly.comp reduces arbitrary words in its first stage,
     # including sums that wrap past 2^64.
     n = 1 << logn
-    q = _prime(logn)
+    q = _prime(logn, bits)
     w = O.root_2n(q, n)
     t = V.NttTables(n, q, w)
     rng = np.random.default_rng(logn)
@@ -200,8 +203,16 @@ def test_ntt_noncanonical_inputs_match_reference(ctx, logn):
     a = vec(ctx, x)
     a.forward_transform(a, t)
     assert np.array_equal(a.read(), O.ntt_forward(x, q, w))
+    # canonical words with a few large ones: waves of both kinds in a launch
+    y = O.fill_uniform(n, q, seed=logn + bits)
+    pos = rng.integers(0, n, 3)
+    y[pos] = np.uint64(U64MAX) - rng.integers(0, 1 << 40, 3, dtype=np.uint64)
+    b = vec(ctx, y)
+    b.forward_transform(b, t)
+    assert np.array_equal(b.read(), O.ntt_forward(y, q, w))
     t.close()
     a.close()
+    b.close()
 
 
 @pytest.mark.parametrize("logn", [2, 3, 5, 7])

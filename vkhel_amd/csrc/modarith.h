@@ -25,11 +25,26 @@ __device__ __forceinline__ uint32_t opaque32(uint32_t v) {
 	return v;
 }
 
+// Same for a wave-uniform value kept in scalar registers.
+__device__ __forceinline__ uint64_t opaque_uniform(uint64_t v) {
+	asm("" : "+s"(v));
+	return v;
+}
+
 // x - m if x >= m else x: one 64-bit subtraction whose borrow selects the
 // result (sub, sub-with-borrow, two selects).
 __device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) {
 	uint64_t d;
 	return __builtin_sub_overflow(x, m, &d) ? x : d;
+}
+
+// csub with nm = 2^64 - m given: the difference is one 64-bit add, and the
+// select a compare (four instructions instead of five).  nm must come from
+// opaque_uniform(), or the add is folded back into a subtraction.
+__device__ __forceinline__ uint64_t csub_n(uint64_t x, uint64_t m,
+		uint64_t nm) {
+	const uint64_t d = x + nm;
+	return x < m ? x : d;
 }
 
 // (a - b) mod q for a, b in [0, q).
@@ -51,29 +66,38 @@ __device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w,
 	return csub(shoup_lazy(y, w, wp, q), q);
 }
 
-// Shoup product with a 63-bit companion wp = floor(w * 2^63 / q), on
-// 32-bit halves.  For y < 2^63 and w < q < 2^61 it returns
-// R = y*w - Q*q, Q = floor(y * wp / 2^63), and R lies in [0, 2q):
+// Shoup product with a 63-bit companion wp = floor(w * 2^63 / q) = p1*2^32
+// + p0, on 32-bit halves.  The companion is passed packed as
+// wpk = (2*p1) << 32 | p0 (shoup63_factor), since 2*p1 is what the
+// quotient needs and p1 = (wpk >> 33) is one shift.
+// For y < 2^63 and w < q < 2^61 it returns R = y*w - Q*q,
+// Q = floor(y * wp / 2^63), and R lies in [0, 2q):
 // y*w/q - Q < 1 + y*(w*2^63/q - wp)/2^63 < 2.
-// Because y1 = y >> 32 and p1 = wp >> 32 are both below 2^31, the middle
+// Because y1 = y >> 32 and p1 are both below 2^31, the middle
 // partial-product sum S = y0*p1 + y1*p0 + floor(y0*p0 / 2^32) stays below
 // 2^64, so Q = 2*y1*p1 + floor(S / 2^31) exactly, in four multiply-adds and
 // one shift, with no carry chain.  R is formed modulo 2^64 as
 // y*w + Q*nq with nq = 2^64 - q.
 __device__ __forceinline__ uint64_t shoup63(uint64_t y, uint64_t w,
-		uint64_t wp, uint64_t nq) {
+		uint64_t wpk, uint64_t nq) {
 	const uint32_t y0 = (uint32_t) y, y1 = (uint32_t) (y >> 32);
-	const uint32_t p0 = (uint32_t) wp, p1 = (uint32_t) (wp >> 32);
+	const uint32_t p0 = (uint32_t) wpk, p1x2 = (uint32_t) (wpk >> 32);
+	const uint32_t p1 = p1x2 >> 1;
 	// (the opaque() points keep each sum a single multiply-add)
 	const uint64_t s0 = opaque((uint64_t) y0 * p1 + __umulhi(y0, p0));
 	const uint64_t s = (uint64_t) y1 * p0 + s0;
-	const uint64_t qt = (uint64_t) y1 * (opaque32(p1) << 1) + (s >> 31);
+	const uint64_t qt = (uint64_t) y1 * p1x2 + (s >> 31);
 	const uint32_t w0 = (uint32_t) w, w1 = (uint32_t) (w >> 32);
 	const uint32_t n0 = (uint32_t) nq, n1 = (uint32_t) (nq >> 32);
 	const uint32_t q0 = (uint32_t) qt, q1 = (uint32_t) (qt >> 32);
 	const uint64_t r = (uint64_t) y0 * w0 + (uint64_t) q0 * n0;
-	const uint32_t rh = opaque32((uint32_t) (r >> 32) + y0 * w1 + y1 * w0 +
-			q0 * n1 + q1 * n0);
+	// Cross terms of the high word, summed in the low halves of a chain of
+	// 64-bit multiply-adds (their high halves are never used).
+	uint64_t c = opaque((uint64_t) y0 * w1);
+	c = opaque((uint64_t) y1 * w0 + c);
+	c = opaque((uint64_t) q0 * n1 + c);
+	c = (uint64_t) q1 * n0 + c;
+	const uint32_t rh = opaque32((uint32_t) (r >> 32) + (uint32_t) c);
 	return ((uint64_t) rh << 32) | (uint32_t) r;
 }
 

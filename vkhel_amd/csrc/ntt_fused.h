@@ -36,16 +36,16 @@ constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
 // below 4q < 2^63 (q < 2^61), as shoup63 requires.  Twiddle pairs hold
 // {w, floor(w * 2^63 / q)}.
 struct LazyK {
-	uint64_t two_q, nq;  // 2q, 2^64 - q
+	uint64_t two_q, n_two_q, nq;  // 2q, 2^64 - 2q, 2^64 - q
 };
 
 __device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
-	return LazyK{c.two_q, 0 - c.q};
+	return LazyK{c.two_q, opaque_uniform(0 - c.two_q), opaque_uniform(0 - c.q)};
 }
 
 __device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
 		const LazyK &k) {
-	const uint64_t xr = csub(x, k.two_q);
+	const uint64_t xr = csub_n(x, k.two_q, k.n_two_q);
 	const uint64_t t = shoup63(y, w.x, w.y, k.nq);
 	x = xr + t;
 	y = xr + k.two_q - t;
@@ -53,7 +53,7 @@ __device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
 
 __device__ __forceinline__ void bf_inv(uint64_t &x, uint64_t &y, ulonglong2 w,
 		const LazyK &k) {
-	const uint64_t s = csub(x + y, k.two_q);
+	const uint64_t s = csub_n(x + y, k.two_q, k.n_two_q);
 	const uint64_t d = x + k.two_q - y;
 	x = s;
 	y = shoup63(d, w.x, w.y, k.nq);
@@ -245,10 +245,10 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 			else
 				v[e] = lds[pad(l)];
 		}
-		// the first round's twiddles; later rounds' were issued before the
-		// previous round's barrier
-		if (FIRST_ROUND)
-			load_round_tw<G, INV, R, LOGN>(w, t, gbase, tw);
+		// every twiddle of the round, issued before any arithmetic (loading
+		// the next round's before the barrier instead costs registers beyond
+		// eight waves per SIMD and measured no faster)
+		load_round_tw<G, INV, R, LOGN>(w, t, gbase, tw);
 	}
 
 	// ---- butterflies ----
@@ -299,7 +299,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		if (!INV && LAST_ROUND && final) {
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				v[e] = csub(csub(v[e], lk.two_q), c.q);
+				v[e] = csub_n(csub_n(v[e], lk.two_q, lk.n_two_q), c.q, lk.nq);
 		}
 	}
 
@@ -316,9 +316,6 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 #pragma unroll
 			for (int e = 0; e < 8; e++)
 				lds[pad(l0 + (e << F))] = v[e];
-			// next round's twiddles, in flight across the barrier
-			if constexpr (!LAST_ROUND)
-				load_round_tw<G, INV, R + 1, LOGN>(w, t, gbase, tw);
 		}
 		__syncthreads();
 		if (LAST_ROUND) {
@@ -396,14 +393,16 @@ __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 // One pass of a two-pass size, one workgroup per (tile, polynomial):
 // blockIdx.x = tile, blockIdx.y = polynomial.  The first pass of a
 // direction reads `in`, the second reads and writes `out`.
-#ifndef VKHEL_PASS_WAVES
-#define VKHEL_PASS_WAVES 8
-#endif
-// Eight waves per SIMD (at most 64 VGPRs): the passes are latency bound at
-// lower occupancy.
+// Waves per SIMD the pass kernels are compiled for: eight (at most 64
+// VGPRs), which every shape fits without spilling except pass 1 of 2^20 on
+// 2^13-word tiles (ten stages over eight columns), which needs about 96.
+constexpr int pass_waves(int logn, int pass, int tb) {
+	return logn == 20 && pass == 1 && tb == 13 ? 5 : 8;
+}
+
 template <int LOGN, int PASS, bool INV, int TB>
 __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
-__attribute__((amdgpu_waves_per_eu(VKHEL_PASS_WAVES))) void k_ntt_pass(
+__attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass(
 		NttTask uniform, const NttTask *tasks, uint32_t second) {
 	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
 	if constexpr (LOGN > TB) {

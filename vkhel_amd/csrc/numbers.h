@@ -20,9 +20,12 @@ inline uint64_t barrett_factor(uint64_t f, uint64_t q) {
 	return (uint64_t) (((u128) f << 64) / q);
 }
 
-// floor(f * 2^63 / q): the 63-bit Shoup companion the device kernels use.
+// The 63-bit Shoup companion wp = floor(f * 2^63 / q) as the device
+// kernels take it: high word doubled, (2 * (wp >> 32)) << 32 | (wp mod 2^32).
+// wp < 2^63 for f < q, so the doubled word fits.
 inline uint64_t shoup63_factor(uint64_t f, uint64_t q) {
-	return barrett_factor(f, q) >> 1;
+	const uint64_t wp = barrett_factor(f, q) >> 1;
+	return wp + (wp & 0xffffffff00000000ull);
 }
 
 // Barrett constant floor(2^(bits+62) / q) of elemmul.c:159-162 and

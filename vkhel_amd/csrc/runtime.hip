@@ -64,13 +64,13 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 		return it->second;
 
 	const uint64_t n = t->n, q = t->q;
-	// {w, floor(w * 2^63 / q)} pairs for both directions.
+	// {w, packed 63-bit Shoup companion} pairs for both directions.
 	std::vector<ulonglong2> host(2 * n);
 	for (uint64_t i = 0; i < n; i++) {
 		host[i] = make_ulonglong2(t->roots_of_unity[i],
-				t->roots_barrett_factors[i] >> 1);
+				shoup63_factor(t->roots_of_unity[i], q));
 		host[n + i] = make_ulonglong2(t->inv_roots_of_unity[i],
-				t->inv_roots_barrett_factors[i] >> 1);
+				shoup63_factor(t->inv_roots_of_unity[i], q));
 	}
 	DevTables *d = new DevTables;
 	d->n = n;
