@@ -8,8 +8,10 @@ as one replayed hipGraph on the library's stream.  `value` is transform-
 coefficients per second (2 * 512 * n per step) with inputs resident in HBM.
 
 Also reported, each with its own timing:
-  * roofline of the dominant kernel (fused forward NTT at n = 2^16), timed
-    with HIP events on the library stream;
+  * roofline of the dominant kernel, the forward NTT of the batch at
+    n = 2^16 (two pass dispatches of k_ntt_pass), timed with HIP events on
+    the library stream, with the HBM traffic and VALU occupancy of the
+    committed rocprofv3 summary of the same workload;
   * plain_per_call: the same transforms through the synchronous reference
     API (vkhel_vector_forward_transform / _inverse_transform), one call per
     polynomial;
@@ -474,16 +476,30 @@ def config1(V, ctx):
             "gpu_per_call_ms": round(gpu_s * 1e3, 3), "bit_exact": ok}
 
 
-def traffic_from_profile(logn, npoly):
-    """HBM bytes per launch of the forward kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary for this workload (scaled by
-    the copy-kernel calibration recorded beside it), or None."""
-    path = os.path.join(REPO, "profiles", f"ntt_fused_fwd_n{1 << logn}_x{npoly}.json")
+def traffic_from_profile(logn, npoly, tile_bits):
+    """HBM bytes per forward transform of the batch (both pass dispatches),
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (scaled by the copy-kernel calibration recorded beside it), or
+    None when no summary of the same tile size exists."""
+    path = os.path.join(REPO, "profiles", f"ntt_fwd_n{1 << logn}_x{npoly}.json")
     try:
         with open(path) as f:
-            return json.load(f)["hbm_bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
-        return None
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if prof.get("tile_bits") != tile_bits:
+        return None, prof
+    return prof.get("hbm_bytes_per_launch"), prof
+
+
+def fwd_kernel_names(logn, tile_bits, pass_mode):
+    """The dispatches of one forward transform at this size and tuning."""
+    if logn <= tile_bits:
+        return [f"k_ntt_fused<{logn}, false, {tile_bits}>"]
+    if pass_mode == 1:
+        return [f"k_ntt_fused<{logn}, false, {tile_bits}>"]
+    return [f"k_ntt_pass<{logn}, 1, false, {tile_bits}>",
+            f"k_ntt_pass<{logn}, 2, false, {tile_bits}>"]
 
 
 def main():
@@ -523,6 +539,7 @@ def main():
     t_tune = time.perf_counter()
     ctx.autotune(12, 17)
     tuned = {l: ctx.tile_bits(l) for l in range(12, 18)}
+    modes = {l: ctx.pass_mode(l) for l in range(12, 18)}
     t_tune = time.perf_counter() - t_tune
 
     r = batched_ntt(V, ctx, args.logn, args.primes, args.per_prime, args.steps,
@@ -555,6 +572,8 @@ def main():
         extra["config1"] = config1(V, ctx)
 
     cpu = None if args.no_cpu else cpu_baseline(args.logn, qs, ws)
+    tb = ctx.tile_bits(args.logn)
+    traffic, prof = traffic_from_profile(args.logn, npoly, tb)
     out = {
         "metric": "batched NTT Gcoeff/s & %HBM peak at N=2^16; served p50/p99 vs plain per-call",
         "value": round(r["gcoeff"], 3),
@@ -574,12 +593,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(args.logn, npoly),
-                     "kernel": "k_ntt_fused<16, fwd>", "kernel_ms": round(r["fwd_ms"], 4),
+                     "traffic": traffic,
+                     "kernel": "forward NTT of the batch: " + " + ".join(
+                         fwd_kernel_names(args.logn, tb, ctx.pass_mode(args.logn))),
+                     "kernel_ms": round(r["fwd_ms"], 4),
                      "inverse_kernel_ms": round(r["inv_ms"], 4),
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "profile": (f"profiles/ntt_fwd_n{n}_x{npoly}.md" if prof else None),
+                     "profiled_valu_busy": prof.get("valu_busy") if prof and traffic else None},
         "plain_per_call": plain,
         "tile_bits": tuned,
+        "pass_mode": modes,
         "autotune_s": round(t_tune, 2),
         "cpu_baseline": cpu,
     }
