@@ -164,12 +164,12 @@ bool ntt_fused_supported(uint32_t logn) {
 }
 
 // Untuned default tile per size, from batched timings on MI355X (2^25 words
-// per batch, split passes): 2^11 tiles up to n = 2^15, 2^12 at 2^16, 2^13
-// above; sizes up to 2^13 run in one pass on a 2^13 tile.
+// per batch, split passes): 2^12 tiles from n = 2^14 to 2^17, 2^13 above;
+// sizes up to 2^13 run in one pass on a 2^13 tile.
 static int default_tile_bits(uint32_t logn) {
 	if (logn <= 13)
 		return 13;
-	return logn <= 15 ? 11 : logn == 16 ? 12 : 13;
+	return logn <= 17 ? 12 : 13;
 }
 
 int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn) {

@@ -127,17 +127,47 @@ struct Geo {
 	static constexpr int NTILES = PASS == 0 ? 1 : 1 << (LOGN - TB);
 };
 
-// Field position and stage range of round R (forward: top-down; inverse:
-// bottom-up).
+// Rounds of a pass: 3-stage groups over local bits [LO, HI), forward
+// top-down, inverse bottom-up.  Bit 9 is where rounds stop being
+// wave-local (the exchange rule below), so the groups are cut at bit 9 when
+// that needs no extra round: then at most the rounds touching bits >= 9
+// exchange across waves.
+constexpr int ceil3(int x) {
+	return (x + 2) / 3;
+}
+
+template <class G>
+struct RoundPlan {
+	static constexpr int M = G::LO > 9 ? G::LO : G::HI < 9 ? G::HI : 9;
+	static constexpr int NU = ceil3(G::HI - M);  // groups above the cut
+	static constexpr int NL = ceil3(M - G::LO);  // groups below it
+	static constexpr bool CUT = NU + NL == ceil3(G::HI - G::LO);
+	static constexpr int ROUNDS = ceil3(G::HI - G::LO);
+};
+
+// Field position and stage range of round R.
 template <class G, bool INV, int R>
 struct Round {
+	using P = RoundPlan<G>;
+	// plain partition
 	static constexpr int BHI_F = G::HI - 3 * R;
 	static constexpr int BLO_F = BHI_F - 3 >= G::LO ? BHI_F - 3 : G::LO;
 	static constexpr int BLO_I = G::LO + 3 * R;
 	static constexpr int BHI_I = BLO_I + 3 <= G::HI ? BLO_I + 3 : G::HI;
-	static constexpr int BLO = INV ? BLO_I : BLO_F;
-	static constexpr int BHI = INV ? BHI_I : BHI_F;
-	static constexpr int F = BLO + 3 <= G::KT ? BLO : G::KT - 3;
+	// partition cut at bit P::M
+	static constexpr int CF_HI = R < P::NU ? G::HI - 3 * R : P::M - 3 * (R - P::NU);
+	static constexpr int CF_FLOOR = R < P::NU ? P::M : G::LO;
+	static constexpr int CF_LO = CF_HI - 3 >= CF_FLOOR ? CF_HI - 3 : CF_FLOOR;
+	static constexpr int CI_LO = R < P::NL ? G::LO + 3 * R : P::M + 3 * (R - P::NL);
+	static constexpr int CI_CEIL = R < P::NL ? P::M : G::HI;
+	static constexpr int CI_HI = CI_LO + 3 <= CI_CEIL ? CI_LO + 3 : CI_CEIL;
+	static constexpr int BLO = P::CUT ? (INV ? CI_LO : CF_LO) : (INV ? BLO_I : BLO_F);
+	static constexpr int BHI = P::CUT ? (INV ? CI_HI : CF_HI) : (INV ? BHI_I : BHI_F);
+	// The 3-bit field holding the round's stages: at BLO (or as high as the
+	// tile allows), or at bit 6 when a short round can sit there entirely
+	// below bit 9 and so stay inside each wave.
+	static constexpr int F0 = BLO + 3 <= G::KT ? BLO : G::KT - 3;
+	static constexpr int F = F0 > 6 && BHI - 3 <= 6 ? 6 : F0;
 	// Lanes of a wave are contiguous in memory for 2^RUN words; direct
 	// global access is used when that covers a 128-byte line.
 	static constexpr int RUN = F < G::CBITS ? F : G::CBITS;
@@ -152,12 +182,18 @@ typedef __attribute__((address_space(1))) const uint64_t gcu64;
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) const u64x2 gcu128;
 
+// Word indices within one polynomial stay below 2^28, so the byte offset
+// is formed in 32 bits: the compiler then addresses with the wave-uniform
+// base and a 32-bit per-lane offset instead of a 64-bit address per lane.
+typedef __attribute__((address_space(1))) const char gcchar;
+typedef __attribute__((address_space(1))) char gchar;
+
 __device__ __forceinline__ uint64_t gload(const uint64_t *p, uint32_t i) {
-	return ((gcu64 *) p)[i];
+	return *(gcu64 *) ((gcchar *) p + (i << 3));
 }
 
 __device__ __forceinline__ void gstore(uint64_t *p, uint32_t i, uint64_t v) {
-	((gu64 *) p)[i] = v;
+	*(gu64 *) ((gchar *) p + (i << 3)) = v;
 }
 
 __device__ __forceinline__ ulonglong2 gload_tw(const ulonglong2 *p,
@@ -167,12 +203,13 @@ __device__ __forceinline__ ulonglong2 gload_tw(const ulonglong2 *p,
 }
 
 // One padding word per 8 keeps the radix-8 access patterns conflict free.
-__device__ __forceinline__ uint32_t pad(uint32_t l) {
+__host__ __device__ __forceinline__ uint32_t pad(uint32_t l) {
 	return l + (l >> 3);
 }
 
 template <class G>
-__device__ __forceinline__ uint32_t tile_addr(uint32_t l, uint32_t jbase) {
+__host__ __device__ __forceinline__ uint32_t tile_addr(uint32_t l,
+		uint32_t jbase) {
 	if (G::CBITS >= G::KT)
 		return jbase + l;
 	return ((l >> G::CBITS) << G::JSHIFT) + (l & ((1u << G::CBITS) - 1)) +
@@ -212,7 +249,70 @@ __device__ __forceinline__ void load_round_tw(ulonglong2 w[7], uint32_t t,
 	}
 }
 
-template <class G, bool INV, int R, int LOGN>
+// LDS exchanges.  In a round whose 3-bit field sits at local bit F <= 6, the
+// 64 lanes of wave w hold exactly the 512 indices [512w, 512w + 512) (lane
+// bits fill index bits below 9 around the field, wave bits are index bits 9
+// and up).  An exchange between two such rounds therefore stays inside each
+// wave and needs no workgroup barrier: LDS operations of one wave complete in
+// program order, so a compiler-level wave fence suffices.  Each thread only
+// rewrites the LDS words it read in the same round, so no exchange can
+// overwrite words another wave has still to read.
+constexpr int kWaveLocalMaxField = 6;
+
+__device__ __forceinline__ void wave_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool CROSS>
+__device__ __forceinline__ void exchange_sync() {
+	if constexpr (CROSS)
+		__syncthreads();
+	else
+		wave_sync();
+}
+
+// Lane-contiguous (staged) global access of a tile: thread t moves words
+// base(t) + 64k, k = 0..7, with base(t) = 512*(t/64) + t%64, so each wave
+// touches only its own 512-word block and each instruction 64 consecutive
+// words.  Bits 6-8 of base(t) are zero, so the padded LDS index and the
+// global index of word k are the base's plus a constant; the offsets are
+// spelled out because the compiler cannot see that the adds do not carry.
+template <class G>
+struct Staged {
+	static constexpr bool WAVE = G::THREADS >= 64;
+	static_assert(G::CBITS >= G::KT || G::CBITS <= 6, "column run above a wave");
+	static constexpr uint32_t PAD_STEP = 72;  // pad(x + 64) - pad(x)
+	static constexpr uint32_t G_STEP = G::CBITS >= G::KT ? 64 :
+		(64u >> G::CBITS) << G::JSHIFT;
+
+	__host__ __device__ static uint32_t base(uint32_t t) {
+		return ((t >> 6) << 9) | (t & 63);
+	}
+	// padded LDS index and global index of word k of thread t
+	__host__ __device__ static uint32_t lds_at(uint32_t t, int k) {
+		if constexpr (WAVE)
+			return pad(base(t)) + k * PAD_STEP;
+		else
+			return pad(t + k * G::THREADS);
+	}
+	__host__ __device__ static uint32_t global_at(uint32_t t, int k,
+			uint32_t jbase) {
+		if constexpr (WAVE)
+			return tile_addr<G>(base(t), jbase) + k * G_STEP;
+		else
+			return tile_addr<G>(t + k * G::THREADS, jbase);
+	}
+};
+
+template <class G, bool INV, int R>
+constexpr bool round_wave_local() {
+	return Round<G, INV, R>::F <= kWaveLocalMaxField;
+}
+
+// TRAIL: end with a workgroup barrier, for callers that reuse the LDS tile.
+template <class G, bool INV, int R, int LOGN, bool TRAIL>
 __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		uint32_t t, const uint64_t *src, uint64_t *dst, uint64_t *lds,
 		uint32_t jbase, uint32_t gbase, const ulonglong2 *tw, const NttConsts &c,
@@ -221,6 +321,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	constexpr int F = RD::F;
 	constexpr bool FIRST_ROUND = R == 0;
 	constexpr bool LAST_ROUND = R == G::ROUNDS - 1;
+	constexpr bool LOCAL = round_wave_local<G, INV, R>();
 	const uint32_t l0 = round_base<F>(t);
 	const bool active = t < (uint32_t) G::THREADS;
 
@@ -229,12 +330,11 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		// lane-contiguous global loads staged through LDS
 		if (active) {
 #pragma unroll
-			for (int k = 0; k < 8; k++) {
-				const uint32_t l = t + k * G::THREADS;
-				lds[pad(l)] = gload(src, tile_addr<G>(l, jbase));
-			}
+			for (int k = 0; k < 8; k++)
+				lds[Staged<G>::lds_at(t, k)] = gload(src,
+						Staged<G>::global_at(t, k, jbase));
 		}
-		__syncthreads();
+		exchange_sync<!LOCAL>();
 	}
 	if (active) {
 #pragma unroll
@@ -304,42 +404,49 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	}
 
 	// ---- results ----
-	if (LAST_ROUND && RD::DIRECT) {
+	if constexpr (LAST_ROUND && RD::DIRECT) {
 		if (active) {
 #pragma unroll
 			for (int e = 0; e < 8; e++)
 				gstore(dst, tile_addr<G>(l0 + (e << F), jbase), v[e]);
 		}
-		__syncthreads();
+		if constexpr (TRAIL)
+			__syncthreads();
+	} else if constexpr (LAST_ROUND) {
+		// lane-contiguous global stores staged through LDS
+		if (active) {
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+				lds[pad(l0 + (e << F))] = v[e];
+		}
+		exchange_sync<!LOCAL>();
+		if (active) {
+#pragma unroll
+			for (int k = 0; k < 8; k++)
+				gstore(dst, Staged<G>::global_at(t, k, jbase),
+						lds[Staged<G>::lds_at(t, k)]);
+		}
+		if constexpr (TRAIL)
+			__syncthreads();
 	} else {
 		if (active) {
 #pragma unroll
 			for (int e = 0; e < 8; e++)
 				lds[pad(l0 + (e << F))] = v[e];
 		}
-		__syncthreads();
-		if (LAST_ROUND) {
-			if (active) {
-#pragma unroll
-				for (int k = 0; k < 8; k++) {
-					const uint32_t l = t + k * G::THREADS;
-					gstore(dst, tile_addr<G>(l, jbase), lds[pad(l)]);
-				}
-			}
-			__syncthreads();
-		}
+		exchange_sync<!(LOCAL && round_wave_local<G, INV, R + 1>())>();
 	}
 }
 
-template <class G, bool INV, int LOGN, int... R>
+template <class G, bool INV, int LOGN, bool TRAIL, int... R>
 __device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...>,
 		uint32_t t, const uint64_t *src, uint64_t *dst, uint64_t *lds,
 		uint32_t jbase, uint32_t gbase, const ulonglong2 *tw, const NttConsts &c,
 		bool final) {
 	uint64_t v[8];
 	ulonglong2 w[7];
-	(run_round<G, INV, R, LOGN>(v, w, t, src, dst, lds, jbase, gbase, tw, c,
-			final), ...);
+	(run_round<G, INV, R, LOGN, TRAIL>(v, w, t, src, dst, lds, jbase, gbase, tw,
+			c, final), ...);
 }
 
 template <int LOGN, int PASS, int TB, bool INV>
@@ -352,8 +459,8 @@ __device__ __forceinline__ void run_pass(const uint64_t *src, uint64_t *dst,
 		// i (2^TB contiguous words), which is also their twiddle offset.
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
-		run_rounds<G, INV, LOGN>(std::make_integer_sequence<int, G::ROUNDS>(), t,
-				src, dst, lds, jbase, gbase, tw, c, final);
+		run_rounds<G, INV, LOGN, true>(std::make_integer_sequence<int, G::ROUNDS>(),
+				t, src, dst, lds, jbase, gbase, tw, c, final);
 	}
 }
 
@@ -419,9 +526,10 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 		const uint32_t i = blockIdx.x;
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
-		run_rounds<G, INV, LOGN>(std::make_integer_sequence<int, G::ROUNDS>(),
-				threadIdx.x, second ? k.out : k.in, k.out, lds, jbase, gbase, k.tw,
-				k.c, second != 0);
+		run_rounds<G, INV, LOGN, false>(
+				std::make_integer_sequence<int, G::ROUNDS>(), threadIdx.x,
+				second ? k.out : k.in, k.out, lds, jbase, gbase, k.tw, k.c,
+				second != 0);
 	}
 }
 
