@@ -285,7 +285,9 @@ def test_batch_mixed_jobs_vs_oracle(ctx):
 
 
 def test_polymul_matches_schoolbook(ctx):
-    for logn in (4, 8, 11):
+    # n = 2, 4: separate pointwise phase; larger n: the fused inverse forms
+    # the product of the two spectra as it loads them
+    for logn in (1, 2, 4, 8, 11):
         n = 1 << logn
         q = O.primes_ntt(1 << 55, 2 * n, 1)[0]
         w = O.root_2n(q, n)
@@ -298,6 +300,35 @@ def test_polymul_matches_schoolbook(ctx):
         assert np.array_equal(r.read(), O.negacyclic_schoolbook(xa, xb, q))
         batch.close()
         t.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("logn", [12, 13, 14, 16])
+def test_polymul_rns_vs_oracle(ctx, logn, mode):
+    """Three-limb products (50-, 60- and 61-bit primes) in both pass modes,
+    against the oracle's NTT-based product (itself pinned to the schoolbook
+    product in the oracle tests)."""
+    n = 1 << logn
+    qs = [O.primes_ntt(1 << bits, 2 * n, 1)[0] for bits in (50, 60, 61)]
+    tabs = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+    xa = np.concatenate([O.fill_uniform(n, q, 10 + i) for i, q in enumerate(qs)])
+    xb = np.concatenate([O.fill_uniform(n, q, 20 + i) for i, q in enumerate(qs)])
+    a, b, r = vec(ctx, xa), vec(ctx, xb), ctx.vector(3 * n)
+    ctx.set_pass_mode(logn, mode)
+    try:
+        batch = V.Batch(ctx)
+        batch.add(V.JOB_POLYMUL, a, b, r, tabs)
+        batch.run()
+        got = r.read()
+        for i, q in enumerate(qs):
+            want = O.polymul_ntt(xa[i * n:(i + 1) * n], xb[i * n:(i + 1) * n], q,
+                                 tabs[i].w)
+            assert np.array_equal(got[i * n:(i + 1) * n], want), (logn, i)
+        batch.close()
+    finally:
+        ctx.set_pass_mode(logn, 0)
+        for t in tabs:
+            t.close()
 
 
 def test_batch_roundtrip_full_size(ctx):

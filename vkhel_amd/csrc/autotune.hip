@@ -1,9 +1,10 @@
 // Autotuner for the fused NTT kernels.  For each transform size it times
 // every compiled tile size (2^11, 2^12, 2^13 words, i.e. the stage-fusion
-// split and workgroup shape of the two-pass path) on a synthetic batch held
-// in device memory, and keeps the fastest forward+inverse pair for the
-// context.  The reference has no counterpart: its kernels have one fixed
+// split and workgroup shape of the two-pass path) in both pass modes on a
+// synthetic batch held in device memory, in interleaved rounds, and keeps
+// the fastest forward+inverse pair for the context.  The reference has no counterpart: its kernels have one fixed
 // shape (64-thread workgroups, one dispatch per butterfly group).
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -27,6 +28,7 @@ __global__ void k_fill_residues(uint64_t *p, uint64_t n, uint64_t q,
 }
 
 static const int kTileChoices[] = {11, 12, 13};
+static const int kTuneRounds = 4;
 
 }  // namespace vk
 
@@ -118,44 +120,62 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 		t.c = c;
 		const int saved = ctx->tile_bits[logn];
 		const int saved_mode = ctx->pass_mode[logn];
-		float best = 1e30f;
-		int best_tb = tile_bits_for(ctx, logn), best_mode = 0;
+		// Candidates: tile 2^11 / 2^12 / 2^13 x whole-polynomial / split
+		// passes (one-tile sizes have a single kernel).  Each is timed in
+		// several interleaved rounds and keeps its fastest, so a transient
+		// slowdown of the device does not decide the choice.
+		int cand_tb[6], cand_mode[6], ncand = 0;
 		for (int choice = 0; choice < 6; choice++) {
 			const int tb = kTileChoices[choice % 3];
 			const int mode = 1 + choice / 3;  // whole polynomial, then split
 			if (logn <= (uint32_t) tb && (tb != 13 || mode == 2))
 				continue;  // one-tile size: every table holds the same kernel
-			ctx->tile_bits[logn] = tb;
-			ctx->pass_mode[logn] = mode;
-			int rc = VKHEL_OK;
-			for (int rep = 0; rep < 2 && rc == VKHEL_OK; rep++) {  // warm-up
+			cand_tb[ncand] = tb;
+			cand_mode[ncand++] = mode;
+		}
+		float cand_ms[6];
+		for (int i = 0; i < ncand; i++)
+			cand_ms[i] = 1e30f;
+		int rc = VKHEL_OK;
+		for (int round = 0; round < kTuneRounds && rc == VKHEL_OK; round++) {
+			for (int i = 0; i < ncand && rc == VKHEL_OK; i++) {
+				ctx->tile_bits[logn] = cand_tb[i];
+				ctx->pass_mode[logn] = cand_mode[i];
+				// warm-up
 				rc = launch_ntt(ctx, t, data, data, polys, false, s);
 				if (rc == VKHEL_OK)
 					rc = launch_ntt(ctx, t, data, data, polys, true, s);
-			}
-			(void) hipEventRecord(e0, s);
-			const int reps = 5;
-			for (int rep = 0; rep < reps && rc == VKHEL_OK; rep++) {
-				rc = launch_ntt(ctx, t, data, data, polys, false, s);
-				if (rc == VKHEL_OK)
-					rc = launch_ntt(ctx, t, data, data, polys, true, s);
-			}
-			(void) hipEventRecord(e1, s);
-			float ms = 0;
-			if (rc != VKHEL_OK || !hip_ok(hipEventSynchronize(e1), "sync") ||
-					!hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed")) {
-				ctx->tile_bits[logn] = saved;
-				ctx->pass_mode[logn] = saved_mode;
-				(void) hipFree(data);
-				(void) hipFree(tw);
-				return vkhel_ext_last_error();
-			}
-			if (ms < best) {
-				best = ms;
-				best_tb = tb;
-				best_mode = mode;
+				(void) hipEventRecord(e0, s);
+				const int reps = 3;
+				for (int rep = 0; rep < reps && rc == VKHEL_OK; rep++) {
+					rc = launch_ntt(ctx, t, data, data, polys, false, s);
+					if (rc == VKHEL_OK)
+						rc = launch_ntt(ctx, t, data, data, polys, true, s);
+				}
+				(void) hipEventRecord(e1, s);
+				float ms = 0;
+				if (rc != VKHEL_OK || !hip_ok(hipEventSynchronize(e1), "sync") ||
+						!hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed")) {
+					rc = rc != VKHEL_OK ? rc : vkhel_ext_last_error();
+					break;
+				}
+				cand_ms[i] = std::min(cand_ms[i], ms);
 			}
 		}
+		if (rc != VKHEL_OK) {
+			ctx->tile_bits[logn] = saved;
+			ctx->pass_mode[logn] = saved_mode;
+			(void) hipEventDestroy(e0);
+			(void) hipEventDestroy(e1);
+			(void) hipFree(data);
+			(void) hipFree(tw);
+			return rc;
+		}
+		int best = 0;
+		for (int i = 1; i < ncand; i++)
+			if (cand_ms[i] < cand_ms[best])
+				best = i;
+		const int best_tb = cand_tb[best], best_mode = cand_mode[best];
 		ctx->tile_bits[logn] = best_tb;
 		ctx->pass_mode[logn] = best_mode;
 		(void) hipEventDestroy(e0);

@@ -5,8 +5,10 @@
 // each a handful of launches over every polynomial limb of every job:
 //
 //   1. forward transforms   (NTT_FWD jobs, both operands of POLYMUL jobs)
-//   2. pointwise products   (MUL, ADD jobs, the spectra of POLYMUL jobs)
-//   3. inverse transforms   (NTT_INV jobs, POLYMUL results)
+//   2. pointwise products   (MUL, ADD jobs; POLYMUL spectra only for sizes
+//                            the fused kernels do not cover)
+//   3. inverse transforms   (NTT_INV jobs, POLYMUL results; the fused
+//                            inverse multiplies the two spectra as it loads)
 //
 // Transform tasks are grouped by size: one launch per distinct n, each
 // workgroup taking one polynomial limb.  The batch API captures the launch
@@ -159,12 +161,19 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				sp += n;
 				ts.fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
 				ts.fwd.push_back(make_task(*t, j.b->dev + off, tmp, false));
-				PointTask p = mul;
-				p.a = j.res->dev + off;
-				p.b = tmp;
-				p.out = j.res->dev + off;
-				ts.pt.push_back(p);
-				ts.inv.push_back(make_task(*t, j.res->dev + off, j.res->dev + off, true));
+				NttTask inv = make_task(*t, j.res->dev + off, j.res->dev + off, true);
+				if (ntt_fused_supported(t->c.logn)) {
+					// the fused inverse forms the pointwise product as it loads
+					inv.in2 = tmp;
+				} else {
+					PointTask p = mul;
+					p.a = j.res->dev + off;
+					p.b = tmp;
+					p.out = j.res->dev + off;
+					ts.pt.push_back(p);
+					ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
+				}
+				ts.inv.push_back(inv);
 				break;
 			}
 			case VKHEL_JOB_MUL:
@@ -178,7 +187,7 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				break;
 			}
 			}
-			if (j.op >= VKHEL_JOB_POLYMUL)
+			if (j.op >= VKHEL_JOB_MUL)
 				ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
 		}
 	}

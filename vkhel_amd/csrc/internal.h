@@ -102,6 +102,10 @@ struct NttTask {
 	uint64_t *out;
 	const ulonglong2 *tw;
 	NttConsts c;
+	// Inverse transforms only (fused kernels): when set, the transform's
+	// input is the pointwise product in[i] * in2[i] mod q of two canonical
+	// spectra, formed as they are loaded (a polynomial product's last step).
+	const uint64_t *in2;
 };
 
 // Launchers (ntt.hip).  `batch` polynomials of n words, poly p at

@@ -152,6 +152,7 @@ NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
 	k.out = out;
 	k.tw = inverse ? t.inv : t.fwd;
 	k.c = t.c;
+	k.in2 = nullptr;
 	return k;
 }
 

@@ -111,6 +111,7 @@ constexpr int ntt_k1(int logn, int tile_bits) {
 
 template <int LOGN, int PASS, int TB>
 struct Geo {
+	static constexpr int P = PASS;
 	static constexpr int K1 = ntt_k1(LOGN, TB);
 	static constexpr int K2 = LOGN - K1;
 	static constexpr int KT = PASS == 0 ? LOGN : TB;
@@ -306,6 +307,14 @@ struct Staged {
 	}
 };
 
+// The pointwise product of a polynomial product, fused into the inverse
+// transform's first loads: (x * y) mod q for canonical spectra x, y, by the
+// reference's 128-bit Barrett reduction (elemmul.comp:49-60), canonical.
+__device__ __forceinline__ uint64_t spectra_product(uint64_t x, uint64_t y,
+		const NttConsts &c) {
+	return barrett128(mulhi64(x, y), x * y, c.q, c.mu, c.bshift);
+}
+
 template <class G, bool INV, int R>
 constexpr bool round_wave_local() {
 	return Round<G, INV, R>::F <= kWaveLocalMaxField;
@@ -314,9 +323,9 @@ constexpr bool round_wave_local() {
 // TRAIL: end with a workgroup barrier, for callers that reuse the LDS tile.
 template <class G, bool INV, int R, int LOGN, bool TRAIL>
 __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
-		uint32_t t, const uint64_t *src, uint64_t *dst, uint64_t *lds,
-		uint32_t jbase, uint32_t gbase, const ulonglong2 *tw, const NttConsts &c,
-		bool final) {
+		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
+		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
+		const NttConsts &c, bool final) {
 	using RD = Round<G, INV, R>;
 	constexpr int F = RD::F;
 	constexpr bool FIRST_ROUND = R == 0;
@@ -326,24 +335,45 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	const bool active = t < (uint32_t) G::THREADS;
 
 	// ---- operands ----
+	// the first loads of an inverse transform may form a pointwise product
+	constexpr bool PRODUCT = INV && FIRST_ROUND && G::P != 1;
 	if (FIRST_ROUND && !RD::DIRECT) {
 		// lane-contiguous global loads staged through LDS
 		if (active) {
+			if (PRODUCT && src2) {
+				// four products in flight: all eight would not fit in
+				// eight waves' registers
+#pragma unroll 4
+				for (int k = 0; k < 8; k++) {
+					const uint32_t gi = Staged<G>::global_at(t, k, jbase);
+					lds[Staged<G>::lds_at(t, k)] = spectra_product(gload(src, gi),
+							gload(src2, gi), c);
+				}
+			} else {
 #pragma unroll
-			for (int k = 0; k < 8; k++)
-				lds[Staged<G>::lds_at(t, k)] = gload(src,
-						Staged<G>::global_at(t, k, jbase));
+				for (int k = 0; k < 8; k++)
+					lds[Staged<G>::lds_at(t, k)] = gload(src,
+							Staged<G>::global_at(t, k, jbase));
+			}
 		}
 		exchange_sync<!LOCAL>();
 	}
 	if (active) {
+		if (FIRST_ROUND && RD::DIRECT && PRODUCT && src2) {
 #pragma unroll
-		for (int e = 0; e < 8; e++) {
-			const uint32_t l = l0 + (e << F);
-			if (FIRST_ROUND && RD::DIRECT)
-				v[e] = gload(src, tile_addr<G>(l, jbase));
-			else
-				v[e] = lds[pad(l)];
+			for (int e = 0; e < 8; e++) {
+				const uint32_t gi = tile_addr<G>(l0 + (e << F), jbase);
+				v[e] = spectra_product(gload(src, gi), gload(src2, gi), c);
+			}
+		} else {
+#pragma unroll
+			for (int e = 0; e < 8; e++) {
+				const uint32_t l = l0 + (e << F);
+				if (FIRST_ROUND && RD::DIRECT)
+					v[e] = gload(src, tile_addr<G>(l, jbase));
+				else
+					v[e] = lds[pad(l)];
+			}
 		}
 		// every twiddle of the round, issued before any arithmetic (loading
 		// the next round's before the barrier instead costs registers beyond
@@ -440,18 +470,19 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 
 template <class G, bool INV, int LOGN, bool TRAIL, int... R>
 __device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...>,
-		uint32_t t, const uint64_t *src, uint64_t *dst, uint64_t *lds,
-		uint32_t jbase, uint32_t gbase, const ulonglong2 *tw, const NttConsts &c,
-		bool final) {
+		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
+		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
+		const NttConsts &c, bool final) {
 	uint64_t v[8];
 	ulonglong2 w[7];
-	(run_round<G, INV, R, LOGN, TRAIL>(v, w, t, src, dst, lds, jbase, gbase, tw,
-			c, final), ...);
+	(run_round<G, INV, R, LOGN, TRAIL>(v, w, t, src, src2, dst, lds, jbase,
+			gbase, tw, c, final), ...);
 }
 
 template <int LOGN, int PASS, int TB, bool INV>
-__device__ __forceinline__ void run_pass(const uint64_t *src, uint64_t *dst,
-		uint64_t *lds, const ulonglong2 *tw, const NttConsts &c, bool final) {
+__device__ __forceinline__ void run_pass(const uint64_t *src,
+		const uint64_t *src2, uint64_t *dst, uint64_t *lds, const ulonglong2 *tw,
+		const NttConsts &c, bool final) {
 	using G = Geo<LOGN, PASS, TB>;
 	const uint32_t t = threadIdx.x;
 	for (uint32_t i = 0; i < (uint32_t) G::NTILES; i++) {
@@ -460,7 +491,7 @@ __device__ __forceinline__ void run_pass(const uint64_t *src, uint64_t *dst,
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
 		run_rounds<G, INV, LOGN, true>(std::make_integer_sequence<int, G::ROUNDS>(),
-				t, src, dst, lds, jbase, gbase, tw, c, final);
+				t, src, src2, dst, lds, jbase, gbase, tw, c, final);
 	}
 }
 
@@ -485,15 +516,17 @@ __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 		const uint64_t off = (uint64_t) blockIdx.x << LOGN;
 		k.in += off;
 		k.out += off;
+		if (k.in2)
+			k.in2 += off;
 	}
 	if constexpr (LOGN <= TB) {
-		run_pass<LOGN, 0, TB, INV>(k.in, k.out, lds, k.tw, k.c, true);
+		run_pass<LOGN, 0, TB, INV>(k.in, k.in2, k.out, lds, k.tw, k.c, true);
 	} else if constexpr (!INV) {
-		run_pass<LOGN, 1, TB, INV>(k.in, k.out, lds, k.tw, k.c, false);
-		run_pass<LOGN, 2, TB, INV>(k.out, k.out, lds, k.tw, k.c, true);
+		run_pass<LOGN, 1, TB, INV>(k.in, nullptr, k.out, lds, k.tw, k.c, false);
+		run_pass<LOGN, 2, TB, INV>(k.out, nullptr, k.out, lds, k.tw, k.c, true);
 	} else {
-		run_pass<LOGN, 2, TB, INV>(k.in, k.out, lds, k.tw, k.c, false);
-		run_pass<LOGN, 1, TB, INV>(k.out, k.out, lds, k.tw, k.c, true);
+		run_pass<LOGN, 2, TB, INV>(k.in, k.in2, k.out, lds, k.tw, k.c, false);
+		run_pass<LOGN, 1, TB, INV>(k.out, nullptr, k.out, lds, k.tw, k.c, true);
 	}
 }
 
@@ -521,6 +554,8 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 			const uint64_t off = (uint64_t) blockIdx.y << LOGN;
 			k.in += off;
 			k.out += off;
+			if (k.in2)
+				k.in2 += off;
 		}
 		using G = Geo<LOGN, PASS, TB>;
 		const uint32_t i = blockIdx.x;
@@ -528,8 +563,8 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
 		run_rounds<G, INV, LOGN, false>(
 				std::make_integer_sequence<int, G::ROUNDS>(), threadIdx.x,
-				second ? k.out : k.in, k.out, lds, jbase, gbase, k.tw, k.c,
-				second != 0);
+				second ? k.out : k.in, second ? nullptr : k.in2, k.out, lds, jbase,
+				gbase, k.tw, k.c, second != 0);
 	}
 }
 
