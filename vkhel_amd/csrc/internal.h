@@ -97,6 +97,16 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t);
 // One polynomial transform for the fused kernel: which words, which
 // direction's twiddles, the per-(n, q) constants and the pass split
 // (k1 = stages in pass A; 0 for n <= 2^13).
+// Device twiddle tables keep the reference's per-stage blocks (stage with m
+// groups reads roots[m + i], src/vector.c:536-547) but store each block of m >= 4
+// entries 4-way interleaved: entry 4j + g of the block at g * (m/4) + j.
+// In the fused kernels' innermost stages a thread uses four consecutive
+// entries, so with this order each wave-wide load reads one contiguous run
+// instead of 16 bytes out of every 64.
+__host__ __device__ __forceinline__ uint32_t tw_pos(uint32_t m, uint32_t i) {
+	return m < 4 ? m + i : m + (i & 3) * (m >> 2) + (i >> 2);
+}
+
 struct NttTask {
 	const uint64_t *in;
 	uint64_t *out;

@@ -25,7 +25,14 @@ __device__ __forceinline__ uint32_t opaque32(uint32_t v) {
 	return v;
 }
 
-// Same for a wave-uniform value kept in scalar registers.
+// A value the compiler must treat as produced here: quantities derived from
+// it are computed after this point instead of being hoisted and kept live.
+__device__ __forceinline__ uint32_t opaque32_here(uint32_t v) {
+	asm volatile("" : "+v"(v));
+	return v;
+}
+
+// Same as opaque() for a wave-uniform value kept in scalar registers.
 __device__ __forceinline__ uint64_t opaque_uniform(uint64_t v) {
 	asm("" : "+s"(v));
 	return v;

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void k_fwd_stage(const uint64_t *in,
 		const uint64_t grp = b >> hb, pos = b & (((uint64_t) 1 << hb) - 1);
 		const uint64_t x = (poly << c.logn) + (grp << (hb + 1)) + pos;
 		const uint64_t y = x + ((uint64_t) 1 << hb);
-		const ulonglong2 w = tw[((uint64_t) 1 << s) + grp];
+		const ulonglong2 w = tw[tw_pos(1u << s, (uint32_t) grp)];
 		const uint64_t X = in[x], Y = in[y];
 		// twiddle pairs hold 63-bit Shoup companions (inputs below 2^63)
 		const uint64_t wy = csub(shoup63(FIRST ? barrett64(Y, c.q, c.mu,
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_inv_stage(const uint64_t *in,
 			out[x] = csub(shoup63(sum, c.ninv, c.ninv_p, 0 - c.q), c.q);
 			out[y] = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
 		} else {
-			const ulonglong2 w = tw[((uint64_t) 1 << s) + grp];
+			const ulonglong2 w = tw[tw_pos(1u << s, (uint32_t) grp)];
 			out[x] = sum;
 			out[y] = csub(shoup63(d, w.x, w.y, 0 - c.q), c.q);
 		}

@@ -243,10 +243,17 @@ __device__ __forceinline__ void load_round_tw(ulonglong2 w[7], uint32_t t,
 		if (lb < RD::BLO || lb >= RD::BHI)
 			continue;
 		const int b = lb + G::SHIFT;
-		const uint32_t ti = (1u << (LOGN - 1 - b)) + ((gbase + l0) >> (lb + 1));
+		// Group index of the thread's first butterfly at this stage, in a
+		// stage block of M entries.  It is a multiple of the stage's
+		// butterfly-group count 4 >> d (the round field's bits of l0 are
+		// zero, gbase is a multiple of the tile), so the thread's entries sit
+		// at one interleaved position plus multiples of M/4.
+		const uint32_t M = 1u << (LOGN - 1 - b);
+		const uint32_t STRIDE = M < 4 ? 1 : M >> 2;
+		const uint32_t p0 = tw_pos(M, (gbase + l0) >> (lb + 1));
 #pragma unroll
 		for (int gi = 0; gi < (4 >> d); gi++)
-			w[tw_slot(d, gi)] = gload_tw(tw, ti + gi);
+			w[tw_slot(d, gi)] = gload_tw(tw, p0 + gi * STRIDE);
 	}
 }
 
@@ -436,9 +443,11 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	// ---- results ----
 	if constexpr (LAST_ROUND && RD::DIRECT) {
 		if (active) {
+			// store addresses formed here, not held from the kernel's start
+			const uint32_t ls = round_base<F>(opaque32_here(t));
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				gstore(dst, tile_addr<G>(l0 + (e << F), jbase), v[e]);
+				gstore(dst, tile_addr<G>(ls + (e << F), jbase), v[e]);
 		}
 		if constexpr (TRAIL)
 			__syncthreads();
@@ -451,10 +460,11 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		}
 		exchange_sync<!LOCAL>();
 		if (active) {
+			const uint32_t ts = opaque32_here(t);
 #pragma unroll
 			for (int k = 0; k < 8; k++)
-				gstore(dst, Staged<G>::global_at(t, k, jbase),
-						lds[Staged<G>::lds_at(t, k)]);
+				gstore(dst, Staged<G>::global_at(ts, k, jbase),
+						lds[Staged<G>::lds_at(ts, k)]);
 		}
 		if constexpr (TRAIL)
 			__syncthreads();

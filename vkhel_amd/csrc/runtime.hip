@@ -64,13 +64,17 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 		return it->second;
 
 	const uint64_t n = t->n, q = t->q;
-	// {w, packed 63-bit Shoup companion} pairs for both directions.
-	std::vector<ulonglong2> host(2 * n);
-	for (uint64_t i = 0; i < n; i++) {
-		host[i] = make_ulonglong2(t->roots_of_unity[i],
-				shoup63_factor(t->roots_of_unity[i], q));
-		host[n + i] = make_ulonglong2(t->inv_roots_of_unity[i],
-				shoup63_factor(t->inv_roots_of_unity[i], q));
+	// {w, packed 63-bit Shoup companion} pairs for both directions, stage
+	// blocks interleaved as tw_pos() lays them out (entry 0 is unused).
+	std::vector<ulonglong2> host(2 * n, make_ulonglong2(0, 0));
+	for (uint64_t m = 1; m < n; m <<= 1) {
+		for (uint64_t i = 0; i < m; i++) {
+			const uint64_t p = tw_pos((uint32_t) m, (uint32_t) i);
+			host[p] = make_ulonglong2(t->roots_of_unity[m + i],
+					shoup63_factor(t->roots_of_unity[m + i], q));
+			host[n + p] = make_ulonglong2(t->inv_roots_of_unity[m + i],
+					shoup63_factor(t->inv_roots_of_unity[m + i], q));
+		}
 	}
 	DevTables *d = new DevTables;
 	d->n = n;
