@@ -121,14 +121,19 @@ uint64_t job_transform_coeffs(const Job &j) {
 	}
 }
 
-// Sorts tasks by log2 n and records the runs.
+// Sorts tasks by log2 n and modulus class and records the runs.
 static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
+	const auto key = [](const NttTask &k) {
+		return 2 * k.c.logn + (k.c.q < kNoReduceQ ? 1 : 0);
+	};
 	std::stable_sort(tasks.begin(), tasks.end(),
-			[](const NttTask &a, const NttTask &b) { return a.c.logn < b.c.logn; });
+			[&](const NttTask &a, const NttTask &b) { return key(a) < key(b); });
 	std::vector<Segment> segs;
 	for (uint32_t i = 0; i < tasks.size(); i++) {
-		if (segs.empty() || segs.back().logn != tasks[i].c.logn)
-			segs.push_back(Segment{i, 0, tasks[i].c.logn});
+		const bool small_q = tasks[i].c.q < kNoReduceQ;
+		if (segs.empty() || segs.back().logn != tasks[i].c.logn ||
+				segs.back().small_q != small_q)
+			segs.push_back(Segment{i, 0, tasks[i].c.logn, small_q});
 		segs.back().count++;
 	}
 	return segs;
@@ -228,7 +233,8 @@ static int launch_segments(const vkhel_ctx *ctx,
 	for (const Segment &g : segs) {
 		int rc = VKHEL_OK;
 		if (ntt_fused_supported(g.logn)) {
-			rc = launch_fused_tasks(ctx, dev + g.off, g.count, g.logn, inverse, s);
+			rc = launch_fused_tasks(ctx, dev + g.off, g.count, g.logn, inverse,
+					g.small_q, s);
 		} else {
 			for (uint32_t i = 0; i < g.count && rc == VKHEL_OK; i++)
 				rc = launch_task_ntt(host[g.off + i], inverse, s);

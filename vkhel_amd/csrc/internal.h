@@ -34,6 +34,7 @@ struct NttConsts {
 	uint64_t ninv_p;  // floor(ninv * 2^63 / q), the 63-bit Shoup companion
 	uint64_t w1n;     // inv_roots[1] * n^-1 mod q (last inverse stage)
 	uint64_t w1n_p;
+	uint64_t one_p;   // 63-bit Shoup companion of 1 (a reduction mod q)
 	uint32_t bshift;  // bits - 2
 	uint32_t logn;
 };
@@ -107,6 +108,12 @@ __host__ __device__ __forceinline__ uint32_t tw_pos(uint32_t m, uint32_t i) {
 	return m < 4 ? m + i : m + (i & 3) * (m >> 2) + (i >> 2);
 }
 
+// Forward transforms over moduli below this bound run without the
+// butterflies' input reduction: values entering stage s >= 1 stay below
+// (2s + 1) q, i.e. below 41 q < 2^63 for log2 n <= 20, which the 63-bit
+// Shoup product accepts.
+constexpr uint64_t kNoReduceQ = 1ull << 57;
+
 struct NttTask {
 	const uint64_t *in;
 	uint64_t *out;
@@ -125,9 +132,11 @@ int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
 NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
 		bool inverse);
 // Device-resident tasks that share log2 n (which selects the compiled
-// kernel) and direction; the context supplies the tuned tile size.
+// kernel), direction and modulus class (small_q: every q below kNoReduceQ);
+// the context supplies the tuned tile size.
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
-		uint32_t ntasks, uint32_t logn, bool inverse, hipStream_t s);
+		uint32_t ntasks, uint32_t logn, bool inverse, bool small_q,
+		hipStream_t st);
 // Tile size (log2 words) used for log2 n on this context.
 int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn);
 bool ntt_fused_supported(uint32_t logn);

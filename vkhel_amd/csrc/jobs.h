@@ -25,9 +25,11 @@ struct PointTask {
 	uint32_t shift, op;  // op: 0 = mul (elemmul semantics), 1 = add
 };
 
-// A run of transform tasks with one log2 n.
+// A run of transform tasks with one log2 n and one modulus class (small_q:
+// every q below kNoReduceQ).
 struct Segment {
 	uint32_t off, count, logn;
+	bool small_q;
 };
 
 struct TaskSet {

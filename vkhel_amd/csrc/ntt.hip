@@ -184,24 +184,23 @@ int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn) {
 // passes as separate grids of one workgroup per tile.
 static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
 		uint32_t ntasks, uint32_t logn, int tile_bits, bool split, bool inverse,
-		hipStream_t st) {
+		bool small_q, hipStream_t st) {
 	const FusedEntry &e = fused_entry(logn, tile_bits);
+	const bool nr = small_q && !inverse;
 	if (split && e.tiles > 1) {
 		const dim3 grid(e.tiles, ntasks);
-		hipLaunchKernelGGL(inverse ? e.inv_b : e.fwd_a, grid, dim3(e.threads), 0,
-				st, uniform, dev_tasks, 0u);
-		hipLaunchKernelGGL(inverse ? e.inv_a : e.fwd_b, grid, dim3(e.threads), 0,
-				st, uniform, dev_tasks, 1u);
+		hipLaunchKernelGGL(inverse ? e.inv_b : nr ? e.fwd_a_nr : e.fwd_a, grid,
+				dim3(e.threads), 0, st, uniform, dev_tasks, 0u);
+		hipLaunchKernelGGL(inverse ? e.inv_a : nr ? e.fwd_b_nr : e.fwd_b, grid,
+				dim3(e.threads), 0, st, uniform, dev_tasks, 1u);
 	} else {
-		hipLaunchKernelGGL(inverse ? e.inv : e.fwd, dim3(ntasks), dim3(e.threads),
-				0, st, uniform, dev_tasks);
+		hipLaunchKernelGGL(inverse ? e.inv : nr ? e.fwd_nr : e.fwd,
+				dim3(ntasks), dim3(e.threads), 0, st, uniform, dev_tasks);
 	}
 	return hip_ok(hipGetLastError(), "ntt fused launch") ? VKHEL_OK
 		: VKHEL_ERR_DEVICE;
 }
 
-// Batches smaller than this many polynomials use the split passes so that
-// more than one workgroup works on each polynomial.
 // Two-pass sizes run their passes as separate grids (one workgroup per tile)
 // unless tuned otherwise: at every size and batch measured, the split grids'
 // smaller register footprint and full-chip occupancy beat one workgroup per
@@ -212,14 +211,15 @@ static bool use_split(const vkhel_ctx *ctx, uint32_t logn) {
 }
 
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
-		uint32_t ntasks, uint32_t logn, bool inverse, hipStream_t st) {
+		uint32_t ntasks, uint32_t logn, bool inverse, bool small_q,
+		hipStream_t st) {
 	if (ntasks == 0)
 		return VKHEL_OK;
 	if (!fused_supported(logn))
 		return fail(VKHEL_ERR_INVALID, "fused NTT: unsupported log2 n = %u", logn);
 	const NttTask none{};
 	return launch_fused_impl(none, dev_tasks, ntasks, logn,
-			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, st);
+			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, small_q, st);
 }
 
 int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
@@ -230,7 +230,8 @@ int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
 	if (ctx->ntt_algo == VKHEL_NTT_RADIX2 || !fused_supported(logn))
 		return launch_radix2(t, in, out, batch, inverse, s);
 	return launch_fused_impl(make_task(t, in, out, inverse), nullptr, batch, logn,
-			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, s);
+			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse,
+			t.c.q < kNoReduceQ, s);
 }
 
 static void transform(const vkhel_vector *op, vkhel_vector *res,

@@ -20,6 +20,10 @@ typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
 struct FusedEntry {
 	FusedKernel fwd, inv;
 	PassKernel fwd_a, fwd_b, inv_b, inv_a;
+	// forward kernels without the butterflies' input reduction (q below
+	// kNoReduceQ)
+	FusedKernel fwd_nr;
+	PassKernel fwd_a_nr, fwd_b_nr;
 	int threads;
 	int tiles;  // tiles per pass (1: one-tile size, passes unused)
 };
@@ -46,6 +50,16 @@ __device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
 __device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
 		const LazyK &k) {
 	const uint64_t xr = csub_n(x, k.two_q, k.n_two_q);
+	const uint64_t t = shoup63(y, w.x, w.y, k.nq);
+	x = xr + t;
+	y = xr + k.two_q - t;
+}
+
+// Forward butterfly without the input reduction (q < kNoReduceQ): each
+// stage grows the bound by less than 2q.
+__device__ __forceinline__ void bf_fwd_nr(uint64_t &x, uint64_t &y,
+		ulonglong2 w, const LazyK &k) {
+	const uint64_t xr = x;
 	const uint64_t t = shoup63(y, w.x, w.y, k.nq);
 	x = xr + t;
 	y = xr + k.two_q - t;
@@ -328,7 +342,8 @@ constexpr bool round_wave_local() {
 }
 
 // TRAIL: end with a workgroup barrier, for callers that reuse the LDS tile.
-template <class G, bool INV, int R, int LOGN, bool TRAIL>
+// NR: forward butterflies without input reduction (q < kNoReduceQ).
+template <class G, bool INV, int R, int LOGN, bool TRAIL, bool NR>
 __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
 		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
@@ -427,6 +442,8 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 						const int e = (gi << (d + 1)) | li;
 						if (INV)
 							bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+						else if (NR)
+							bf_fwd_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 						else
 							bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 					}
@@ -434,9 +451,16 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 			}
 		}
 		if (!INV && LAST_ROUND && final) {
+			if (NR) {
+				// from below 41q: a Shoup product by 1, then one subtraction
 #pragma unroll
-			for (int e = 0; e < 8; e++)
-				v[e] = csub_n(csub_n(v[e], lk.two_q, lk.n_two_q), c.q, lk.nq);
+				for (int e = 0; e < 8; e++)
+					v[e] = csub_n(shoup63(v[e], 1, c.one_p, lk.nq), c.q, lk.nq);
+			} else {
+#pragma unroll
+				for (int e = 0; e < 8; e++)
+					v[e] = csub_n(csub_n(v[e], lk.two_q, lk.n_two_q), c.q, lk.nq);
+			}
 		}
 	}
 
@@ -478,18 +502,18 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	}
 }
 
-template <class G, bool INV, int LOGN, bool TRAIL, int... R>
+template <class G, bool INV, int LOGN, bool TRAIL, bool NR, int... R>
 __device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...>,
 		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
 		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
 		const NttConsts &c, bool final) {
 	uint64_t v[8];
 	ulonglong2 w[7];
-	(run_round<G, INV, R, LOGN, TRAIL>(v, w, t, src, src2, dst, lds, jbase,
+	(run_round<G, INV, R, LOGN, TRAIL, NR>(v, w, t, src, src2, dst, lds, jbase,
 			gbase, tw, c, final), ...);
 }
 
-template <int LOGN, int PASS, int TB, bool INV>
+template <int LOGN, int PASS, int TB, bool INV, bool NR>
 __device__ __forceinline__ void run_pass(const uint64_t *src,
 		const uint64_t *src2, uint64_t *dst, uint64_t *lds, const ulonglong2 *tw,
 		const NttConsts &c, bool final) {
@@ -500,8 +524,9 @@ __device__ __forceinline__ void run_pass(const uint64_t *src,
 		// i (2^TB contiguous words), which is also their twiddle offset.
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
-		run_rounds<G, INV, LOGN, true>(std::make_integer_sequence<int, G::ROUNDS>(),
-				t, src, src2, dst, lds, jbase, gbase, tw, c, final);
+		run_rounds<G, INV, LOGN, true, NR>(
+				std::make_integer_sequence<int, G::ROUNDS>(), t, src, src2, dst, lds,
+				jbase, gbase, tw, c, final);
 	}
 }
 
@@ -514,7 +539,7 @@ struct FusedShape {
 
 // One workgroup per task.  With `tasks` null, workgroup i transforms the
 // i-th polynomial after `uniform` (same tables, contiguous polynomials).
-template <int LOGN, bool INV, int TB>
+template <int LOGN, bool INV, int TB, bool NR = false>
 __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 		NttTask uniform, const NttTask *tasks) {
 	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
@@ -530,13 +555,16 @@ __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 			k.in2 += off;
 	}
 	if constexpr (LOGN <= TB) {
-		run_pass<LOGN, 0, TB, INV>(k.in, k.in2, k.out, lds, k.tw, k.c, true);
+		run_pass<LOGN, 0, TB, INV, NR>(k.in, k.in2, k.out, lds, k.tw, k.c, true);
 	} else if constexpr (!INV) {
-		run_pass<LOGN, 1, TB, INV>(k.in, nullptr, k.out, lds, k.tw, k.c, false);
-		run_pass<LOGN, 2, TB, INV>(k.out, nullptr, k.out, lds, k.tw, k.c, true);
+		run_pass<LOGN, 1, TB, INV, NR>(k.in, nullptr, k.out, lds, k.tw, k.c,
+				false);
+		run_pass<LOGN, 2, TB, INV, NR>(k.out, nullptr, k.out, lds, k.tw, k.c,
+				true);
 	} else {
-		run_pass<LOGN, 2, TB, INV>(k.in, k.in2, k.out, lds, k.tw, k.c, false);
-		run_pass<LOGN, 1, TB, INV>(k.out, nullptr, k.out, lds, k.tw, k.c, true);
+		run_pass<LOGN, 2, TB, INV, NR>(k.in, k.in2, k.out, lds, k.tw, k.c, false);
+		run_pass<LOGN, 1, TB, INV, NR>(k.out, nullptr, k.out, lds, k.tw, k.c,
+				true);
 	}
 }
 
@@ -550,7 +578,7 @@ constexpr int pass_waves(int logn, int pass, int tb) {
 	return logn == 20 && pass == 1 && tb == 13 ? 5 : 8;
 }
 
-template <int LOGN, int PASS, bool INV, int TB>
+template <int LOGN, int PASS, bool INV, int TB, bool NR = false>
 __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
 __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass(
 		NttTask uniform, const NttTask *tasks, uint32_t second) {
@@ -571,7 +599,7 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 		const uint32_t i = blockIdx.x;
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
-		run_rounds<G, INV, LOGN, false>(
+		run_rounds<G, INV, LOGN, false, NR>(
 				std::make_integer_sequence<int, G::ROUNDS>(), threadIdx.x,
 				second ? k.out : k.in, second ? nullptr : k.in2, k.out, lds, jbase,
 				gbase, k.tw, k.c, second != 0);
@@ -588,6 +616,9 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 		k_ntt_pass<L + kMinFusedLogn, 2, false, TB>,
 		k_ntt_pass<L + kMinFusedLogn, 2, true, TB>,
 		k_ntt_pass<L + kMinFusedLogn, 1, true, TB>,
+		k_ntt_fused<L + kMinFusedLogn, false, TB, true>,
+		k_ntt_pass<L + kMinFusedLogn, 1, false, TB, true>,
+		k_ntt_pass<L + kMinFusedLogn, 2, false, TB, true>,
 		FusedShape<L + kMinFusedLogn, TB>::THREADS,
 		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0)}...};
 }

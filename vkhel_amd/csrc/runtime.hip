@@ -102,6 +102,7 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 	c.ninv_p = shoup63_factor(c.ninv, q);
 	c.w1n = mulmod_exact(t->inv_roots_of_unity[n > 1 ? 1 : 0], c.ninv, q);
 	c.w1n_p = shoup63_factor(c.w1n, q);
+	c.one_p = shoup63_factor(1, q);
 	ctx->tables.emplace(t->id, d);
 	return d;
 }
