@@ -22,6 +22,9 @@ Also reported, each with its own timing:
     ops NTT / INTT / polymul / add / mul, as one batch and per call;
   * serving: the same 1000 jobs through the serving engine at Poisson
     arrival rates, p50/p99 latency beside a FIFO of plain per-call service;
+  * pcie_inclusive: the headline batch with inputs and outputs in host
+    memory (per-call reference API, and a pipelined batch with transfers
+    on copy streams) -- reported beside `value`, never as it;
   * cpu_baseline: the oracle's scalar C NTT on the host cores (bounded
     sample).
 
@@ -430,6 +433,96 @@ def uneven_and_serving(V, ctx, ev, quick=False):
     return uneven, serving
 
 
+def pcie_inclusive(V, ctx, logn=16, primes=8, per_prime=64, chunks=8,
+                   per_call_sample=32):
+    """Host-resident inputs and outputs (the headline batch, n = 2^16,
+    512 polynomials, forward then inverse): transfer time included.
+    (a) reference API, one polynomial at a time: copy_from_host, forward,
+        inverse, read back (a sample of polynomials);
+    (b) pipelined: page-locked buffers, the batch in `chunks` parts, each
+        part's upload and download on their own copy streams overlapping
+        the transforms of the neighbouring parts on the library stream."""
+    from vkhel_amd import hipstream as H
+    n = 1 << logn
+    npoly = primes * per_prime
+    qs = primes_ntt(1 << 50, 2 * n, primes)
+    tabs = [V.NttTables(n, q, root_2n(q, n)) for q in qs]
+    up, down = V.HostBuffer(npoly * n), V.HostBuffer(npoly * n)
+    for i in range(npoly):
+        up.array[i * n:(i + 1) * n] = uniform_mod(qs[i % primes], n, 5000 + i)
+    vecs = [ctx.vector(n, False) for _ in range(npoly)]
+    # (a) reference API per polynomial, pageable host memory
+    host = [np.array(up.array[i * n:(i + 1) * n]) for i in range(per_call_sample)]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(per_call_sample):
+        v = vecs[i]
+        v.copy_from_host(host[i])
+        v.forward_transform(v, tabs[i % primes])
+        v.inverse_transform(v, tabs[i % primes])
+        host[i] = v.read()
+    per_call_s = time.perf_counter() - t0
+    assert all(np.array_equal(host[i], up.array[i * n:(i + 1) * n])
+               for i in range(per_call_sample)), "round trip broken"
+    # (b) pipelined batch
+    per = npoly // chunks
+    batches = []
+    for c in range(chunks):
+        b = V.Batch(ctx)
+        for i in range(c * per, (c + 1) * per):
+            b.add(V.JOB_NTT_FWD, vecs[i], None, vecs[i], [tabs[i % primes]])
+        for i in range(c * per, (c + 1) * per):
+            b.add(V.JOB_NTT_INV, vecs[i], None, vecs[i], [tabs[i % primes]])
+        b.run()
+        batches.append(b)
+    s_up, s_down = H.Stream(), H.Stream()
+    ev_up = [H.Event() for _ in range(chunks)]
+    ev_done = [H.Event() for _ in range(chunks)]
+    start, end = H.Event(), H.Event()
+
+    def run_pipeline():
+        start.record(s_up)
+        for c in range(chunks):
+            for i in range(c * per, (c + 1) * per):
+                vecs[i].write_async(up.array[i * n:(i + 1) * n], 0, s_up)
+            ev_up[c].record(s_up)
+        for c in range(chunks):
+            H.wait(ctx.stream, ev_up[c])
+            batches[c].launch()
+            ev_done[c].record(ctx.stream)
+        for c in range(chunks):
+            s_down.wait(ev_done[c])
+            for i in range(c * per, (c + 1) * per):
+                vecs[i].read_async(down.array[i * n:(i + 1) * n], 0, s_down)
+        end.record(s_down)
+        s_down.synchronize()
+        return start.elapsed_ms(end)
+
+    run_pipeline()  # warm-up
+    ms = min(run_pipeline() for _ in range(3))
+    ok = bool(np.array_equal(down.array, up.array))
+    for e in ev_up + ev_done + [start, end]:
+        e.close()
+    s_up.close()
+    s_down.close()
+    for b in batches:
+        b.close()
+    for v in vecs:
+        v.close()
+    up.close()
+    down.close()
+    for t in tabs:
+        t.close()
+    bytes_moved = 2 * npoly * n * 8
+    return {"workload": f"{npoly} x 2^{logn} forward+inverse, inputs and outputs "
+                        f"in host memory",
+            "pipelined_gcoeff_s": round(2 * npoly * n / (ms * 1e-3) / 1e9, 3),
+            "pipelined_ms": round(ms, 3), "chunks": chunks,
+            "pcie_GBps": round(bytes_moved / (ms * 1e-3) / 1e9, 1),
+            "per_call_gcoeff_s": round(2 * per_call_sample * n / per_call_s / 1e9, 4),
+            "per_call_sample": per_call_sample, "round_trip_exact": ok}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -590,6 +683,7 @@ def main():
         extra["polymul"] = polymul_leg(V, ctx, ev)
         extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
         extra["config1"] = config1(V, ctx)
+        extra["pcie_inclusive"] = pcie_inclusive(V, ctx)
 
     cpu = None if args.no_cpu else cpu_baseline(args.logn, qs, ws)
     tb = ctx.tile_bits(args.logn)

@@ -59,6 +59,21 @@ int vkhel_ext_vector_read(const struct vkhel_vector *, uint64_t offset,
 int vkhel_ext_vector_write(struct vkhel_vector *, uint64_t offset,
 		uint64_t count, const uint64_t *host);
 
+/* ---- asynchronous transfers ---- */
+/* Page-locked host memory (null on failure); free with vkhel_ext_host_free.
+ * Copies from and to it run as DMA without a staging copy and overlap
+ * kernels running on other streams. */
+void *vkhel_ext_host_alloc(size_t bytes);
+void vkhel_ext_host_free(void *);
+/* Enqueue a copy of `count` words at `offset` to / from `host` on `stream`
+ * (a hipStream_t; null: the context's stream) and return without waiting.
+ * The caller orders the copy against kernels on other streams (HIP events)
+ * and keeps `host` alive and untouched until the copy completes. */
+int vkhel_ext_vector_write_async(struct vkhel_vector *, uint64_t offset,
+		uint64_t count, const uint64_t *host, void *stream);
+int vkhel_ext_vector_read_async(const struct vkhel_vector *, uint64_t offset,
+		uint64_t count, uint64_t *host, void *stream);
+
 /* Fills `count` words from `offset` with synthetic residues uniform in
  * [0, q): word i is the high half of splitmix64(seed, i) * q. */
 int vkhel_ext_vector_fill_uniform(struct vkhel_vector *, uint64_t offset,

@@ -419,3 +419,51 @@ def test_autotune_choices_and_cache(ctx, tmp_path):
     for l in chosen:
         ctx.set_tile_bits(l, 0)
         ctx.set_pass_mode(l, 0)
+
+
+# ---------------- asynchronous transfers ----------------
+
+def test_async_transfers_on_a_copy_stream(ctx):
+    """Pinned host buffers, uploads and downloads on a second stream ordered
+    against the library's stream by events: results equal the oracle's."""
+    from vkhel_amd import hipstream as H
+    n, npoly = 1 << 14, 6
+    qs = O.primes_ntt(1 << 50, 2 * n, 2)
+    tabs = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+    xs = [O.fill_uniform(n, qs[i % 2], seed=300 + i) for i in range(npoly)]
+    up, down = V.HostBuffer(npoly * n), V.HostBuffer(npoly * n)
+    for i, x in enumerate(xs):
+        up.array[i * n:(i + 1) * n] = x
+    down.array[:] = 0
+    vecs = [ctx.vector(n, False) for _ in range(npoly)]
+    batch = V.Batch(ctx)
+    for i, v in enumerate(vecs):
+        batch.add(V.JOB_NTT_FWD, v, None, v, [tabs[i % 2]])
+    batch.run()  # plan and capture before timing-free use below
+    copy = H.Stream()
+    uploaded, computed = H.Event(), H.Event()
+    try:
+        for i, v in enumerate(vecs):
+            v.write_async(up.array[i * n:(i + 1) * n], 0, copy)
+        uploaded.record(copy)
+        H.wait(ctx.stream, uploaded)
+        batch.launch()
+        computed.record(ctx.stream)
+        copy.wait(computed)
+        for i, v in enumerate(vecs):
+            v.read_async(down.array[i * n:(i + 1) * n], 0, copy)
+        copy.synchronize()
+        for i, x in enumerate(xs):
+            want = O.ntt_forward(x, qs[i % 2], tabs[i % 2].w)
+            assert np.array_equal(down.array[i * n:(i + 1) * n], want), i
+    finally:
+        copy.close()
+        uploaded.close()
+        computed.close()
+        batch.close()
+        for v in vecs:
+            v.close()
+        up.close()
+        down.close()
+        for t in tabs:
+            t.close()

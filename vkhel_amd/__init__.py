@@ -7,6 +7,7 @@ interface for Python callers and tests; it never falls back to CPU code --
 if the library is missing or no GPU is present, calls raise.
 """
 from .lib import LIB_PATH, VkhelError, load, symbols  # noqa: F401
-from .vkhel import (Batch, Context, NttTables, Server, Vector, JOB_ADD,  # noqa: F401
+from .vkhel import (Batch, Context, HostBuffer, NttTables, Server, Vector,  # noqa: F401
+                    JOB_ADD,
                     JOB_MUL, JOB_NTT_FWD, JOB_NTT_INV, JOB_POLYMUL,
                     NTT_AUTO, NTT_FUSED, NTT_RADIX2)

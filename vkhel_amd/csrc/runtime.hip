@@ -419,6 +419,43 @@ int vkhel_ext_vector_write(struct vkhel_vector *v, uint64_t offset,
 	return VKHEL_OK;
 }
 
+void *vkhel_ext_host_alloc(size_t bytes) {
+	void *p = nullptr;
+	if (!hip_ok(hipHostMalloc(&p, std::max<size_t>(bytes, 1), 0),
+				"hipHostMalloc(host_alloc)"))
+		return nullptr;
+	return p;
+}
+
+void vkhel_ext_host_free(void *p) {
+	if (p)
+		(void) hip_ok(hipHostFree(p), "hipHostFree");
+}
+
+int vkhel_ext_vector_write_async(struct vkhel_vector *v, uint64_t offset,
+		uint64_t count, const uint64_t *host, void *stream) {
+	if (!v || (!host && count) || offset > v->length ||
+			count > v->length - offset)
+		return fail(VKHEL_ERR_INVALID, "vector_write_async: range out of bounds");
+	hipStream_t s = stream ? (hipStream_t) stream : v->ctx->stream;
+	if (count && !hip_ok(hipMemcpyAsync(v->dev + offset, host, count * 8,
+					hipMemcpyHostToDevice, s), "write_async"))
+		return VKHEL_ERR_DEVICE;
+	return VKHEL_OK;
+}
+
+int vkhel_ext_vector_read_async(const struct vkhel_vector *v, uint64_t offset,
+		uint64_t count, uint64_t *host, void *stream) {
+	if (!v || (!host && count) || offset > v->length ||
+			count > v->length - offset)
+		return fail(VKHEL_ERR_INVALID, "vector_read_async: range out of bounds");
+	hipStream_t s = stream ? (hipStream_t) stream : v->ctx->stream;
+	if (count && !hip_ok(hipMemcpyAsync(host, v->dev + offset, count * 8,
+					hipMemcpyDeviceToHost, s), "read_async"))
+		return VKHEL_ERR_DEVICE;
+	return VKHEL_OK;
+}
+
 int vkhel_ext_tables_info(const struct vkhel_ntt_tables *t, uint64_t *n,
 		uint64_t *q, uint64_t *w) {
 	if (!t)

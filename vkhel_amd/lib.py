@@ -56,6 +56,10 @@ SIGNATURES = {
     "vkhel_ext_vector_view": (_p, [_p, _u64, _u64]),
     "vkhel_ext_vector_read": (_int, [_p, _u64, _u64, _pu64]),
     "vkhel_ext_vector_write": (_int, [_p, _u64, _u64, _pu64]),
+    "vkhel_ext_host_alloc": (_p, [ctypes.c_size_t]),
+    "vkhel_ext_host_free": (None, [_p]),
+    "vkhel_ext_vector_write_async": (_int, [_p, _u64, _u64, _pu64, _p]),
+    "vkhel_ext_vector_read_async": (_int, [_p, _u64, _u64, _pu64, _p]),
     "vkhel_ext_tables_info": (_int, [_p, _pu64, _pu64, _pu64]),
     "vkhel_ext_tables_copy": (_int, [_p, _pu64, _pu64, _pu64, _pu64]),
     "vkhel_ext_tables_upload": (_int, [_p, _p]),

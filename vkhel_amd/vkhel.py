@@ -172,6 +172,24 @@ class Vector:
         a = np.ascontiguousarray(arr, dtype=np.uint64)
         _lib.check(_L().vkhel_ext_vector_write(self._h, offset, a.size, _u64p(a)))
 
+    def write_async(self, arr, offset=0, stream=None):
+        """Enqueue a host->device copy of `arr` (ideally a HostBuffer view)
+        on `stream` (a hipstream.Stream; None: the context's stream)."""
+        if not (arr.dtype == np.uint64 and arr.flags.c_contiguous):
+            raise ValueError("write_async needs a contiguous uint64 array")
+        s = stream.handle if stream is not None else None
+        _lib.check(_L().vkhel_ext_vector_write_async(self._h, offset, arr.size,
+                                                     _u64p(arr), s))
+
+    def read_async(self, out, offset=0, stream=None):
+        """Enqueue a device->host copy into `out` (ideally a HostBuffer
+        view) on `stream`; `out` holds the words once the stream passes."""
+        if not (out.dtype == np.uint64 and out.flags.c_contiguous):
+            raise ValueError("read_async needs a contiguous uint64 array")
+        s = stream.handle if stream is not None else None
+        _lib.check(_L().vkhel_ext_vector_read_async(self._h, offset, out.size,
+                                                    _u64p(out), s))
+
     def fill_uniform(self, q, seed, offset=0, count=None):
         """Synthetic residues in [0, q), generated on the device."""
         count = self.length - offset if count is None else count
@@ -228,6 +246,33 @@ class Vector:
         if self._h:
             _L().vkhel_vector_destroy(self._h)
             self._h = None
+
+
+class HostBuffer:
+    """Page-locked host memory (vkhel_ext_host_alloc) of `words` uint64
+    words; `array` is a numpy view of it."""
+
+    def __init__(self, words):
+        L = _L()
+        self._p = L.vkhel_ext_host_alloc(max(words, 1) * 8)
+        if not self._p:
+            _lib.check()
+            raise _lib.VkhelError(-1, "host_alloc failed")
+        self.array = np.ctypeslib.as_array(
+            ctypes.cast(self._p, ctypes.POINTER(ctypes.c_uint64)),
+            shape=(max(words, 1),))[:words]
+
+    def close(self):
+        if self._p:
+            self.array = None
+            _L().vkhel_ext_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
