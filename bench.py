@@ -605,14 +605,14 @@ def traffic_from_profile(logn, npoly, tile_bits):
     return prof.get("hbm_bytes_per_launch"), prof
 
 
-def fwd_kernel_names(logn, tile_bits, pass_mode):
-    """The dispatches of one forward transform at this size and tuning."""
-    if logn <= tile_bits:
-        return [f"k_ntt_fused<{logn}, false, {tile_bits}>"]
-    if pass_mode == 1:
-        return [f"k_ntt_fused<{logn}, false, {tile_bits}>"]
-    return [f"k_ntt_pass<{logn}, 1, false, {tile_bits}>",
-            f"k_ntt_pass<{logn}, 2, false, {tile_bits}>"]
+def fwd_kernel_names(logn, tile_bits, pass_mode, small_q):
+    """The dispatches of one forward transform at this size and tuning
+    (moduli below 2^57 use the kernels without butterfly input reduction)."""
+    nr = ", true" if small_q else ""
+    if logn <= tile_bits or pass_mode == 1:
+        return [f"k_ntt_fused<{logn}, false, {tile_bits}{nr}>"]
+    return [f"k_ntt_pass<{logn}, 1, false, {tile_bits}{nr}>",
+            f"k_ntt_pass<{logn}, 2, false, {tile_bits}{nr}>"]
 
 
 def main():
@@ -710,7 +710,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": "forward NTT of the batch: " + " + ".join(
-                         fwd_kernel_names(args.logn, tb, ctx.pass_mode(args.logn))),
+                         fwd_kernel_names(args.logn, tb, ctx.pass_mode(args.logn),
+                                          max(qs) < (1 << 57))),
                      "kernel_ms": round(r["fwd_ms"], 4),
                      "inverse_kernel_ms": round(r["inv_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes,
