@@ -134,6 +134,12 @@ enum vkhel_job_op {
 	VKHEL_JOB_POLYMUL = 2,  /* res = INTT(NTT(a) . NTT(b)) = a*b mod X^n+1 */
 	VKHEL_JOB_MUL = 3,      /* res = a.b mod q   (elemmul semantics)      */
 	VKHEL_JOB_ADD = 4,      /* res = (a + b) mod q for a, b < q           */
+	/* RNS elementwise ops (vkhel_batch_add_rns), per limb i, q_i from
+	 * tables[i], with the single-call semantics of the named function:   */
+	VKHEL_JOB_CENTER = 5,   /* res_i = elemmod(a_i, params[0], q_i)        */
+	VKHEL_JOB_GTADD = 6,    /* res_i = elemgtadd(a_i, params[2i], params[2i+1]) */
+	VKHEL_JOB_GTSUB = 7,    /* res_i = elemgtsub(a_i, params[3i], params[3i+1],
+	                                             params[3i+2])             */
 };
 
 struct vkhel_batch;
@@ -145,6 +151,15 @@ int vkhel_batch_add(struct vkhel_batch *, int op,
 		const struct vkhel_vector *a, const struct vkhel_vector *b,
 		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,
 		uint32_t nlimbs);
+/* Adds an RNS elementwise job (VKHEL_JOB_CENTER / _GTADD / _GTSUB): every
+ * limb of every such job in the batch runs in the batch's one pointwise
+ * launch.  CENTER is the centered lift of limb i (mod q_i) reduced mod p =
+ * params[0]; GTADD / GTSUB take (bound, diff) / (bound, diff, mod) per
+ * limb.  `params` is copied. */
+int vkhel_batch_add_rns(struct vkhel_batch *, int op,
+		const struct vkhel_vector *a, struct vkhel_vector *res,
+		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs,
+		const uint64_t *params);
 /* Enqueues the batch on the context stream and returns immediately. */
 int vkhel_batch_launch(struct vkhel_batch *);
 /* vkhel_batch_launch + wait. */

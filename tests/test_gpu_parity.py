@@ -467,3 +467,70 @@ def test_async_transfers_on_a_copy_stream(ctx):
         down.close()
         for t in tabs:
             t.close()
+
+
+# ---------------- RNS elementwise batch ops ----------------
+
+@pytest.mark.parametrize("p", [2, 3, 65537, (1 << 40) + 15])
+def test_batch_rns_ops_match_single_calls(ctx, p):
+    """CENTER / GTADD / GTSUB over every limb in one batch launch equal the
+    oracle's elemmod / elemgtadd / elemgtsub per limb and the single calls."""
+    n, L = 1 << 10, 3
+    qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in (40, 50, 60)]
+    tabs = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+    rng = np.random.default_rng(p % 1000)
+    # raw 64-bit words: every branch of the reference arithmetic runs
+    x = rng.integers(0, U64MAX, L * n, dtype=np.uint64, endpoint=True)
+    x[:n] = np.array([O.fill_uniform(n, qs[0], 1)]).ravel()  # limb 0 canonical
+    a = vec(ctx, x)
+    r_center, r_gtadd, r_gtsub = (ctx.vector(L * n) for _ in range(3))
+    gtadd_prm = []
+    gtsub_prm = []
+    for q in qs:
+        gtadd_prm += [q // 2, (1 << 64) - q]
+        gtsub_prm += [q // 2, q, p if p > 2 else 7]
+    batch = V.Batch(ctx)
+    batch.add_rns(V.JOB_CENTER, a, r_center, tabs, [p])
+    batch.add_rns(V.JOB_GTADD, a, r_gtadd, tabs, gtadd_prm)
+    batch.add_rns(V.JOB_GTSUB, a, r_gtsub, tabs, gtsub_prm)
+    batch.run()
+    gc, ga, gs = r_center.read(), r_gtadd.read(), r_gtsub.read()
+    for i, q in enumerate(qs):
+        xi = x[i * n:(i + 1) * n]
+        sl = slice(i * n, (i + 1) * n)
+        assert np.array_equal(gc[sl], O.elemmod(xi, p, q)), ("center", i)
+        assert np.array_equal(ga[sl], O.elemgtadd(xi, gtadd_prm[2 * i],
+                                                  gtadd_prm[2 * i + 1])), ("gtadd", i)
+        assert np.array_equal(gs[sl], O.elemgtsub(xi, *gtsub_prm[3 * i:3 * i + 3])), \
+            ("gtsub", i)
+        # the single-call API on a limb view gives the same words
+        single = ctx.vector(n)
+        a.view(i * n, n).elemmod(single, p, q)
+        assert np.array_equal(single.read(), gc[sl]), ("single center", i)
+        single.close()
+    batch.close()
+    for v in (a, r_center, r_gtadd, r_gtsub):
+        v.close()
+    for t in tabs:
+        t.close()
+
+
+def test_batch_rns_ops_reject_bad_arguments(ctx):
+    n = 1 << 8
+    q = O.primes_ntt(1 << 50, 2 * n, 1)[0]
+    t = V.NttTables(n, q, O.root_2n(q, n))
+    a, r = ctx.vector(n), ctx.vector(n)
+    batch = V.Batch(ctx)
+    with pytest.raises(V.VkhelError):
+        batch.add_rns(V.JOB_CENTER, a, r, [t], [1])       # p < 2
+    with pytest.raises(V.VkhelError):
+        batch.add_rns(V.JOB_GTSUB, a, r, [t], [1, 2, 0])  # mod < 2
+    with pytest.raises(V.VkhelError):
+        batch.add_rns(V.JOB_NTT_FWD, a, r, [t], [0])      # not an RNS op
+    short = ctx.vector(n // 2)
+    with pytest.raises(V.VkhelError):
+        batch.add_rns(V.JOB_GTADD, a, short, [t], [0, 0])
+    batch.close()
+    for v in (a, r, short):
+        v.close()
+    t.close()

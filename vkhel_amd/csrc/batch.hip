@@ -23,37 +23,50 @@
 
 namespace vk {
 
+__device__ __forceinline__ uint64_t point_op(const PointTask &t, uint64_t x,
+		uint64_t y) {
+	switch (t.op) {
+	case kPtMul: {
+		const uint64_t a = barrett64(x, t.q, t.mu, t.shift);
+		const uint64_t b = barrett64(y, t.q, t.mu, t.shift);
+		return barrett128(mulhi64(a, b), a * b, t.q, t.mu, t.shift);
+	}
+	case kPtAdd:
+		return csub(x + y, t.q);
+	case kPtGtSub: {  // elemgtsub.comp:49-67
+		const uint64_t r = barrett64(x, t.q, t.mu, t.shift);
+		if (x > t.bound)
+			return csub(r + t.q - barrett64(t.diff, t.q, t.mu, t.shift), t.q);
+		return r;
+	}
+	case kPtGtAdd:  // elemgtadd.comp:20-30
+		return x > t.bound ? x + t.diff : x;
+	default: {  // kPtModTwo, elemmodbytwo.comp:19-27
+		const uint64_t u = x & 1;
+		return x > t.bound ? 1 - u : u;
+	}
+	}
+}
+
+// One workgroup row per task (blockIdx.y); unary ops never read b.
 __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks) {
 	const PointTask t = tasks[blockIdx.y];
+	const bool binary = t.op == kPtMul || t.op == kPtAdd;
 	const uint64_t pairs = t.len >> 1;
 	const uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
 			i < pairs; i += stride) {
 		const ulonglong2 x = ((const ulonglong2 *) t.a)[i];
-		const ulonglong2 y = ((const ulonglong2 *) t.b)[i];
+		const ulonglong2 y = binary ? ((const ulonglong2 *) t.b)[i]
+			: make_ulonglong2(0, 0);
 		ulonglong2 r;
-		if (t.op == 0) {
-			const uint64_t a0 = barrett64(x.x, t.q, t.mu, t.shift);
-			const uint64_t b0 = barrett64(y.x, t.q, t.mu, t.shift);
-			const uint64_t a1 = barrett64(x.y, t.q, t.mu, t.shift);
-			const uint64_t b1 = barrett64(y.y, t.q, t.mu, t.shift);
-			r.x = barrett128(mulhi64(a0, b0), a0 * b0, t.q, t.mu, t.shift);
-			r.y = barrett128(mulhi64(a1, b1), a1 * b1, t.q, t.mu, t.shift);
-		} else {
-			r.x = csub(x.x + y.x, t.q);
-			r.y = csub(x.y + y.y, t.q);
-		}
+		r.x = point_op(t, x.x, y.x);
+		r.y = point_op(t, x.y, y.y);
 		((ulonglong2 *) t.out)[i] = r;
 	}
 	if ((t.len & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
 		const uint64_t k = t.len - 1;
-		if (t.op == 0) {
-			const uint64_t a0 = barrett64(t.a[k], t.q, t.mu, t.shift);
-			const uint64_t b0 = barrett64(t.b[k], t.q, t.mu, t.shift);
-			t.out[k] = barrett128(mulhi64(a0, b0), a0 * b0, t.q, t.mu, t.shift);
-		} else {
-			t.out[k] = csub(t.a[k] + t.b[k], t.q);
-		}
+		t.out[k] = point_op(t, t.a[k], binary ? t.b[k] : 0);
 	}
 }
 
@@ -95,6 +108,41 @@ int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 	out->b = binary ? b : nullptr;
 	out->res = res;
 	out->tables.assign(tables, tables + nlimbs);
+	return VKHEL_OK;
+}
+
+int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
+		vkhel_vector *res, vkhel_ntt_tables *const *tables, uint32_t nlimbs,
+		const uint64_t *params, Job *out) {
+	if (!a || !res || !tables || nlimbs == 0 || !params)
+		return fail(VKHEL_ERR_INVALID, "rns job: null argument");
+	if (op < VKHEL_JOB_CENTER || op > VKHEL_JOB_GTSUB)
+		return fail(VKHEL_ERR_INVALID, "rns job: unknown op %d", op);
+	if (a->ctx != ctx || res->ctx != ctx)
+		return fail(VKHEL_ERR_INVALID, "rns job: vector from another context");
+	for (uint32_t i = 0; i < nlimbs; i++)
+		if (!tables[i] || tables[i]->n != tables[0]->n)
+			return fail(VKHEL_ERR_INVALID,
+					"rns job: limb tables must all exist and share n");
+	const uint64_t words = (uint64_t) nlimbs * tables[0]->n;
+	if (a->length < words || res->length < words)
+		return fail(VKHEL_ERR_INVALID,
+				"rns job: vectors need %llu words (%u limbs x n)",
+				(unsigned long long) words, nlimbs);
+	const size_t np = op == VKHEL_JOB_CENTER ? 1 :
+		(op == VKHEL_JOB_GTADD ? 2 : 3) * (size_t) nlimbs;
+	if (op == VKHEL_JOB_CENTER && params[0] < 2)
+		return fail(VKHEL_ERR_INVALID, "rns job: center needs p >= 2");
+	if (op == VKHEL_JOB_GTSUB)
+		for (uint32_t i = 0; i < nlimbs; i++)
+			if (params[3 * i + 2] < 2)
+				return fail(VKHEL_ERR_INVALID, "rns job: gtsub limb %u mod < 2", i);
+	out->op = op;
+	out->a = a;
+	out->b = nullptr;
+	out->res = res;
+	out->tables.assign(tables, tables + nlimbs);
+	out->params.assign(params, params + np);
 	return VKHEL_OK;
 }
 
@@ -146,14 +194,17 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 	for (size_t ji = 0; ji < njobs; ji++) {
 		const Job &j = *jobs[ji];
 		const uint64_t n = j.tables[0]->n;
+		// device twiddles only for jobs that transform
+		const bool transforms = j.op <= VKHEL_JOB_POLYMUL;
 		for (size_t l = 0; l < j.tables.size(); l++) {
-			const DevTables *t = device_tables(ctx, j.tables[l]);
-			if (!t)
+			const DevTables *t = transforms ? device_tables(ctx, j.tables[l])
+				: nullptr;
+			if (transforms && !t)
 				return vkhel_ext_last_error();
 			const uint64_t off = l * n;
-			const uint64_t q = t->c.q;
-			const PointTask mul{nullptr, nullptr, nullptr, n, q, barrett_mu(q),
-				(uint32_t) (bitlen(q) - 2), 0};
+			const uint64_t q = j.tables[l]->q;
+			const PointTask mul{nullptr, nullptr, nullptr, n, q, barrett_mu(q), 0, 0,
+				(uint32_t) (bitlen(q) - 2), kPtMul};
 			switch (j.op) {
 			case VKHEL_JOB_NTT_FWD:
 				ts.fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
@@ -187,7 +238,40 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				p.a = j.a->dev + off;
 				p.b = j.b->dev + off;
 				p.out = j.res->dev + off;
-				p.op = j.op == VKHEL_JOB_MUL ? 0 : 1;
+				p.op = j.op == VKHEL_JOB_MUL ? kPtMul : kPtAdd;
+				ts.pt.push_back(p);
+				break;
+			}
+			case VKHEL_JOB_CENTER:
+			case VKHEL_JOB_GTADD:
+			case VKHEL_JOB_GTSUB: {
+				// src/vector.c:342-511: the same per-limb push constants
+				PointTask p{j.a->dev + off, nullptr, j.res->dev + off, n, 0, 0, 0, 0,
+					0, kPtGtAdd};
+				const uint64_t *pr = j.params.data();
+				if (j.op == VKHEL_JOB_CENTER) {
+					const uint64_t pm = pr[0];
+					p.bound = q / 2;
+					if (pm == 2) {
+						p.op = kPtModTwo;
+					} else {
+						p.op = kPtGtSub;
+						p.diff = q;
+						p.q = pm;
+					}
+				} else if (j.op == VKHEL_JOB_GTADD) {
+					p.bound = pr[2 * l];
+					p.diff = pr[2 * l + 1];
+				} else {
+					p.op = kPtGtSub;
+					p.bound = pr[3 * l];
+					p.diff = pr[3 * l + 1];
+					p.q = pr[3 * l + 2];
+				}
+				if (p.op == kPtGtSub) {
+					p.mu = barrett_mu(p.q);
+					p.shift = (uint32_t) (bitlen(p.q) - 2);
+				}
 				ts.pt.push_back(p);
 				break;
 			}
@@ -376,6 +460,21 @@ int vkhel_batch_add(struct vkhel_batch *b, int op, const struct vkhel_vector *a,
 		return fail(VKHEL_ERR_INVALID, "batch_add: null batch");
 	Job j;
 	const int rc = make_job(b->ctx, op, a, bv, res, tables, nlimbs, &j);
+	if (rc != VKHEL_OK)
+		return rc;
+	b->jobs.push_back(std::move(j));
+	b->dirty = true;
+	return VKHEL_OK;
+}
+
+int vkhel_batch_add_rns(struct vkhel_batch *b, int op,
+		const struct vkhel_vector *a, struct vkhel_vector *res,
+		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs,
+		const uint64_t *params) {
+	if (!b)
+		return fail(VKHEL_ERR_INVALID, "batch_add_rns: null batch");
+	Job j;
+	const int rc = make_rns_job(b->ctx, op, a, res, tables, nlimbs, params, &j);
 	if (rc != VKHEL_OK)
 		return rc;
 	b->jobs.push_back(std::move(j));

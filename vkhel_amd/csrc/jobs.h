@@ -15,14 +15,22 @@ struct Job {
 	const vkhel_vector *a, *b;
 	vkhel_vector *res;
 	std::vector<const vkhel_ntt_tables *> tables;
+	std::vector<uint64_t> params;  // RNS elementwise ops only
 };
 
-// One pointwise limb: out = a (op) b over len words mod q.
+// One pointwise limb over len words.  op:
+//   kPtMul   out = a.b mod q (elemmul semantics)
+//   kPtAdd   out = (a + b) mod q
+//   kPtGtSub out = elemgtsub(a, bound, diff, q)  (also elemmod, p != 2)
+//   kPtGtAdd out = elemgtadd(a, bound, diff)
+//   kPtModTwo out = elemmod(a, 2, .) with bound = q_limb / 2
+enum { kPtMul = 0, kPtAdd, kPtGtSub, kPtGtAdd, kPtModTwo };
 struct PointTask {
 	const uint64_t *a, *b;
 	uint64_t *out;
 	uint64_t len, q, mu;
-	uint32_t shift, op;  // op: 0 = mul (elemmul semantics), 1 = add
+	uint64_t bound, diff;
+	uint32_t shift, op;
 };
 
 // A run of transform tasks with one log2 n and one modulus class (small_q:
@@ -51,6 +59,10 @@ struct DeviceTasks {
 int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		const vkhel_vector *b, vkhel_vector *res,
 		vkhel_ntt_tables *const *tables, uint32_t nlimbs, Job *out);
+// Checks an RNS elementwise job (VKHEL_JOB_CENTER / _GTADD / _GTSUB).
+int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
+		vkhel_vector *res, vkhel_ntt_tables *const *tables, uint32_t nlimbs,
+		const uint64_t *params, Job *out);
 // Device scratch words a job needs (the spectrum of b for POLYMUL).
 uint64_t job_scratch_words(const Job &j);
 // Device bytes a job touches (operands, result, scratch): the serving

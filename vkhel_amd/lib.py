@@ -84,6 +84,8 @@ SIGNATURES = {
     "vkhel_batch_destroy": (None, [_p]),
     "vkhel_batch_clear": (None, [_p]),
     "vkhel_batch_add": (_int, [_p, _int, _p, _p, _p, ctypes.POINTER(_p), _u32]),
+    "vkhel_batch_add_rns": (_int, [_p, _int, _p, _p, ctypes.POINTER(_p), _u32,
+                                   _pu64]),
     "vkhel_batch_launch": (_int, [_p]),
     "vkhel_batch_run": (_int, [_p]),
     "vkhel_batch_num_jobs": (_u64, [_p]),
