@@ -38,8 +38,7 @@ __device__ __forceinline__ uint64_t opaque_uniform(uint64_t v) {
 	return v;
 }
 
-// x - m if x >= m else x: one 64-bit subtraction whose borrow selects the
-// result (sub, sub-with-borrow, two selects).
+// x - m if x >= m else x (a compare, a 64-bit subtraction, two selects).
 __device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) {
 	uint64_t d;
 	return __builtin_sub_overflow(x, m, &d) ? x : d;
@@ -60,19 +59,6 @@ __device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b,
 	return a >= b ? a - b : a + q - b;
 }
 
-// Shoup product: y * w - floor(y * wp / 2^64) * q lies in [0, 2q) for any
-// 64-bit y when w < q and wp = floor(w * 2^64 / q) (the reference's
-// nttfwdbutterfly.comp:44-48 arithmetic before its final subtraction).
-__device__ __forceinline__ uint64_t shoup_lazy(uint64_t y, uint64_t w,
-		uint64_t wp, uint64_t q) {
-	return y * w - mulhi64(y, wp) * q;
-}
-
-__device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w,
-		uint64_t wp, uint64_t q) {
-	return csub(shoup_lazy(y, w, wp, q), q);
-}
-
 // Shoup product with a 63-bit companion wp = floor(w * 2^63 / q) = p1*2^32
 // + p0, on 32-bit halves.  The companion is passed packed as
 // wpk = (2*p1) << 32 | p0 (shoup63_factor), since 2*p1 is what the
@@ -82,8 +68,8 @@ __device__ __forceinline__ uint64_t shoup(uint64_t y, uint64_t w,
 // y*w/q - Q < 1 + y*(w*2^63/q - wp)/2^63 < 2.
 // Because y1 = y >> 32 and p1 are both below 2^31, the middle
 // partial-product sum S = y0*p1 + y1*p0 + floor(y0*p0 / 2^32) stays below
-// 2^64, so Q = 2*y1*p1 + floor(S / 2^31) exactly, in four multiply-adds and
-// one shift, with no carry chain.  R is formed modulo 2^64 as
+// 2^64, so Q = 2*y1*p1 + floor(S / 2^31) exactly, from one high multiply,
+// three multiply-adds and one shift, with no carry chain.  R is formed modulo 2^64 as
 // y*w + Q*nq with nq = 2^64 - q.
 __device__ __forceinline__ uint64_t shoup63(uint64_t y, uint64_t w,
 		uint64_t wpk, uint64_t nq) {

@@ -1,21 +1,24 @@
 // Negacyclic NTT / INTT of vkhel (src/vector.c:513-657) for gfx950.
 //
 // The reference issues one dispatch per butterfly group (n - 1 dispatches
-// and log2(n) fenced submits per transform).  Here the whole butterfly
-// network of a polynomial runs inside one workgroup:
+// and log2(n) fenced submits per transform).  Here (ntt_fused.h):
 //
-//   * n <= 2^13: the polynomial is staged once in LDS and every stage runs
-//     there, three stages per register round (radix-8 per thread).
-//   * n >  2^13: two tiled passes.  Pass A runs the top k1 stages on
-//     2^k1-row x 2^cb-column tiles, pass B the low k2 stages on 2^rb-row x
-//     2^k2 tiles (k1 + k2 = log2 n, every tile 2^13 words).  Pass B reads
-//     what pass A wrote through the same workgroup, so __syncthreads() is
-//     the only synchronisation and the intermediate stays in L2/MALL.
+//   * n <= 2^tile: one workgroup per polynomial stages it once in LDS and
+//     runs every stage there, three stages per register round (radix-8 per
+//     thread).
+//   * larger n: two tiled passes, each its own grid of one workgroup per
+//     (tile, polynomial): pass 1 runs the top floor(log2 n / 2) stages on
+//     column tiles, pass 2 the low stages on row tiles.  Tiles of 2^11,
+//     2^12 or 2^13 words are chosen per size (autotuned, or a measured
+//     default).  Whole-polynomial workgroups running both passes remain as
+//     a tunable mode.
 //
-// Twiddles come from the HBM table in the reference's order (roots[m + i],
-// src/ntt_tables.c), as {w, w'} Shoup pairs.  Results are canonical and
+// Twiddles come from device tables in the reference's stage blocks
+// (roots[m + i], src/vector.c:536-547; interleaved within each block, see
+// tw_pos) as {w, 63-bit Shoup companion} pairs.  Results are canonical and
 // equal the reference's stage-by-stage output bit for bit: every butterfly
 // is exact mod q, so only the order of independent butterflies changes.
+// Sizes above 2^20 and the RADIX2 algorithm use the radix-2 kernels below.
 #include <array>
 #include <utility>
 

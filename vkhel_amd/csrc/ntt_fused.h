@@ -16,7 +16,8 @@ typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
 
 // One compiled (log2 n, tile size) variant: whole-transform kernels for both
 // directions (one workgroup per polynomial), the two passes as separate
-// kernels (one workgroup per tile, for small batches), and launch shape.
+// kernels (one workgroup per tile: the default for two-pass sizes), and
+// launch shape.
 struct FusedEntry {
 	FusedKernel fwd, inv;
 	PassKernel fwd_a, fwd_b, inv_b, inv_a;
@@ -37,8 +38,8 @@ constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
 // Inverse (Gentleman-Sande): x, y in [0, 2q) -> [0, 2q).  Both stay
 // congruent mod q to the reference's canonical butterflies
 // (nttfwdbutterfly.comp:44-57, nttrevbutterfly.comp:44-57).  Values stay
-// below 4q < 2^63 (q < 2^61), as shoup63 requires.  Twiddle pairs hold
-// {w, floor(w * 2^63 / q)}.
+// below 4q < 2^63 (q < 2^61), as shoup63 requires.  Twiddle pairs hold w
+// and its packed 63-bit Shoup companion (shoup63_factor).
 struct LazyK {
 	uint64_t two_q, n_two_q, nq;  // 2q, 2^64 - 2q, 2^64 - q
 };
@@ -118,7 +119,7 @@ __device__ __forceinline__ void bf_inv_last(uint64_t &x, uint64_t &y,
 //   PASS 2: 2^rb rows x 2^k2 contiguous words, the low k2 stages.
 // Everything below is fixed at compile time by (LOGN, PASS, TB), TB being
 // the tile size (log2 words) of the two-pass path.
-// Pass A takes the top floor(log2 n / 2) stages (0 for one-tile sizes).
+// Pass 1 takes the top floor(log2 n / 2) stages (none for one-tile sizes).
 constexpr int ntt_k1(int logn, int tile_bits) {
 	return logn <= tile_bits ? 0 : logn / 2;
 }
