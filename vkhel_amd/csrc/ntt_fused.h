@@ -119,9 +119,14 @@ __device__ __forceinline__ void bf_inv_last(uint64_t &x, uint64_t &y,
 //   PASS 2: 2^rb rows x 2^k2 contiguous words, the low k2 stages.
 // Everything below is fixed at compile time by (LOGN, PASS, TB), TB being
 // the tile size (log2 words) of the two-pass path.
-// Pass 1 takes the top floor(log2 n / 2) stages (none for one-tile sizes).
+// Pass 1 takes the top log2 n - 9 stages, so that pass 2 works on rows of
+// 512 words (one wave per row, three full rounds), when pass 1's tiles still
+// have column runs of at least 16 words (128 bytes); otherwise the top
+// floor(log2 n / 2) stages.  None for one-tile sizes.
 constexpr int ntt_k1(int logn, int tile_bits) {
-	return logn <= tile_bits ? 0 : logn / 2;
+	return logn <= tile_bits ? 0
+		: (logn - 9 >= 1 && tile_bits - (logn - 9) >= 4 && logn - 9 < logn / 2)
+			? logn - 9 : logn / 2;
 }
 
 template <int LOGN, int PASS, int TB>
@@ -299,16 +304,21 @@ __device__ __forceinline__ void exchange_sync() {
 // Lane-contiguous (staged) global access of a tile: thread t moves words
 // base(t) + 64k, k = 0..7, with base(t) = 512*(t/64) + t%64, so each wave
 // touches only its own 512-word block and each instruction 64 consecutive
-// words.  Bits 6-8 of base(t) are zero, so the padded LDS index and the
-// global index of word k are the base's plus a constant; the offsets are
-// spelled out because the compiler cannot see that the adds do not carry.
+// words.  base(t) and 64k occupy disjoint bits (base's bits 6-8 are zero),
+// and both the padding and the tile's column split act on disjoint bit
+// fields additively, so the padded LDS index and the global index of word k
+// are the base's plus a constant per k; the offsets are spelled out because
+// the compiler cannot see that the adds do not carry.
 template <class G>
 struct Staged {
 	static constexpr bool WAVE = G::THREADS >= 64;
-	static_assert(G::CBITS >= G::KT || G::CBITS <= 6, "column run above a wave");
 	static constexpr uint32_t PAD_STEP = 72;  // pad(x + 64) - pad(x)
-	static constexpr uint32_t G_STEP = G::CBITS >= G::KT ? 64 :
-		(64u >> G::CBITS) << G::JSHIFT;
+	// tile_addr of 64k without the tile base
+	static constexpr uint32_t g_off(int k) {
+		const uint32_t x = 64u * (uint32_t) k;
+		return G::CBITS >= G::KT ? x :
+			((x >> G::CBITS) << G::JSHIFT) + (x & ((1u << G::CBITS) - 1));
+	}
 
 	__host__ __device__ static uint32_t base(uint32_t t) {
 		return ((t >> 6) << 9) | (t & 63);
@@ -323,7 +333,7 @@ struct Staged {
 	__host__ __device__ static uint32_t global_at(uint32_t t, int k,
 			uint32_t jbase) {
 		if constexpr (WAVE)
-			return tile_addr<G>(base(t), jbase) + k * G_STEP;
+			return tile_addr<G>(base(t), jbase) + g_off(k);
 		else
 			return tile_addr<G>(t + k * G::THREADS, jbase);
 	}
