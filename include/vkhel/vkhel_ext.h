@@ -80,13 +80,35 @@ int vkhel_ext_vector_fill_uniform(struct vkhel_vector *, uint64_t offset,
 		uint64_t count, uint64_t q, uint64_t seed);
 
 /* ---- tables ---- */
-/* Reads back the reference-layout host tables (n words each). */
+/* Reads back the reference-layout host tables (n words each), computing
+ * them first for tables made by vkhel_ext_tables_create_device. */
 int vkhel_ext_tables_info(const struct vkhel_ntt_tables *, uint64_t *n,
 		uint64_t *q, uint64_t *w);
 int vkhel_ext_tables_copy(const struct vkhel_ntt_tables *, uint64_t *roots,
 		uint64_t *inv_roots, uint64_t *roots_bf, uint64_t *inv_roots_bf);
-/* Uploads the tables to `ctx` now instead of at first use. */
+/* Builds the device copy of the tables on `ctx` now instead of at first
+ * use.  Device copies are generated on the GPU from (n, q, w); nothing
+ * crosses PCIe. */
 int vkhel_ext_tables_upload(struct vkhel_ctx *, struct vkhel_ntt_tables *);
+/* Tables built on the GPU of `ctx` only (src/ntt_tables.c:65-80 builds
+ * them on the host, O(n) per table): the host arrays that
+ * vkhel_ext_tables_copy returns are computed on first request.  Unlike
+ * vkhel_ntt_tables_create, w must be a primitive 2n-th root of unity mod q
+ * (w^n = -1), else VKHEL_ERR_INVALID.  Release with
+ * vkhel_ntt_tables_destroy. */
+struct vkhel_ntt_tables *vkhel_ext_tables_create_device(struct vkhel_ctx *,
+		uint64_t n, uint64_t q, uint64_t w);
+/* 1 if the tables' w is a primitive 2n-th root of unity mod q, 0 if not
+ * (the reference accepts any invertible w and never checks); -1 with the
+ * error set for a null handle. */
+int vkhel_ext_tables_root_ok(const struct vkhel_ntt_tables *);
+/* Reads the device copy on `ctx` back in the reference's index order
+ * (n words each; any pointer may be NULL): roots[k] and inv_roots[k] as in
+ * the host arrays, with the packed 63-bit Shoup companions the kernels use,
+ * (2 * (wp >> 32)) << 32 | (wp mod 2^32) for wp = floor(root * 2^63 / q). */
+int vkhel_ext_tables_read_device(struct vkhel_ctx *,
+		const struct vkhel_ntt_tables *, uint64_t *roots,
+		uint64_t *roots_shoup, uint64_t *inv_roots, uint64_t *inv_roots_shoup);
 
 /* ---- NTT kernel selection ---- */
 enum vkhel_ntt_algo {

@@ -95,6 +95,30 @@ def test_tables_match_oracle_at_config_sizes(n, bound):
     tab.close()
 
 
+@pytest.mark.parametrize("bound", [1 << 20, 1 << 50, 1 << 60])
+def test_table_chain_is_exact_powers(bound):
+    """roots[bitrev(k)] = w^k exactly: the reference's repeated Barrett
+    products are canonical below q < 2^61, which is what lets the device
+    build every entry independently from binary powers of w."""
+    n = 1 << 9
+    for q in O.primes_ntt(bound, 2 * n, 3):
+        w = O.root_2n(q, n)
+        r, ir, _, _ = _tables(n, q, w).arrays()
+        rev = [int(format(k, "09b")[::-1], 2) for k in range(n)]
+        assert [int(r[rev[k]]) for k in range(n)] == [pow(w, k, q) for k in range(n)]
+        assert [int(ir[rev[k]]) for k in range(n)] == [pow(w, -k, q) for k in range(n)]
+
+
+def test_tables_root_check():
+    n = 1 << 8
+    q = O.primes_ntt(1 << 40, 2 * n, 1)[0]
+    w = O.root_2n(q, n)
+    assert _tables(n, q, w).root_ok
+    assert pow(w, n, q) == q - 1
+    for bad in (1, w * w % q, q - 1):
+        assert not _tables(n, q, bad).root_ok
+
+
 @pytest.mark.parametrize("n,q,w", [
     (3, 113, 18),                 # n not a power of two
     (1, 113, 18),                 # n < 2

@@ -534,3 +534,68 @@ def test_batch_rns_ops_reject_bad_arguments(ctx):
     for v in (a, r, short):
         v.close()
     t.close()
+
+
+# ---------------- on-device table generation ----------------
+
+def _packed_shoup63(r, q):
+    wp = (int(r) << 63) // q
+    return wp + (wp & 0xFFFFFFFF00000000)
+
+
+@pytest.mark.parametrize("n,q,w", [
+    (2, 5, 2), (8, 17, 3), (8, 17, 1),           # tiny moduli, w = 1 included
+    (1 << 10, None, None), (1 << 16, None, None), (1 << 17, None, None),
+])
+def test_device_tables_match_host_tables(ctx, n, q, w):
+    """Tables built on the GPU equal the reference's host arrays (any
+    invertible w, primitive or not) and carry exact Shoup companions."""
+    if q is None:
+        q = O.primes_ntt(1 << 60 if n > 1 << 16 else 1 << 50, 2 * n, 1)[0]
+        w = O.root_2n(q, n)
+    tab = V.NttTables(n, q, w)
+    roots, roots_p, inv, inv_p = tab.read_device(ctx)
+    want = O.tables(n, q, w)
+    assert np.array_equal(roots, want[0])
+    assert np.array_equal(inv, want[1])
+    idx = range(n) if n <= 1024 else list(range(64)) + list(range(n - 64, n)) + \
+        list(np.random.default_rng(n).integers(0, n, 2000))
+    for k in idx:
+        assert int(roots_p[k]) == _packed_shoup63(roots[k], q), k
+        assert int(inv_p[k]) == _packed_shoup63(inv[k], q), k
+    tab.close()
+
+
+def test_device_created_tables_validate_w_and_transform_alike(ctx):
+    n = 1 << 12
+    q = O.primes_ntt(1 << 55, 2 * n, 1)[0]
+    w = O.root_2n(q, n)
+    # w^2 has order n, and 1 has order 1: neither is a primitive 2n-th root.
+    for bad in (1, w * w % q):
+        with pytest.raises(V.VkhelError) as e:
+            V.NttTables.on_device(ctx, n, q, bad)
+        assert e.value.code == 1
+    host = V.NttTables(n, q, w)
+    dev = V.NttTables.on_device(ctx, n, q, w)
+    assert dev.root_ok and host.root_ok
+    assert not V.NttTables(n, q, w * w % q).root_ok
+    # host arrays of a device-made table are computed on request
+    for g, e in zip(dev.arrays(), O.tables(n, q, w)):
+        assert np.array_equal(g, e)
+    rng = np.random.default_rng(7)
+    a = rng.integers(0, q, n, dtype=np.uint64)
+    outs = []
+    for tab in (host, dev):
+        v = ctx.from_array(a)
+        f = V.Vector(ctx, n)
+        b = V.Vector(ctx, n)
+        v.forward_transform(f, tab)
+        f.inverse_transform(b, tab)
+        outs.append((f.read(), b.read()))
+        for x in (v, f, b):
+            x.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[1][0], O.ntt_forward(a, q, w))
+    assert np.array_equal(outs[1][1], a)
+    host.close()
+    dev.close()

@@ -63,11 +63,17 @@ struct vkhel_ntt_tables {
 	uint64_t *roots_barrett_factors;
 	uint64_t *inv_roots_barrett_factors;
 	uint64_t id;
+	// 1 if w is a primitive 2n-th root of unity mod q (w^n = -1), which the
+	// reference never checks; set by the create calls.
+	int root_ok;
 };
 
 struct vkhel_ctx {
 	int device = 0;
 	hipStream_t stream = nullptr;
+	// Private stream for table generation: never captured, so generating
+	// a table is safe while another thread captures on `stream`.
+	hipStream_t table_stream = nullptr;
 	std::string device_name;
 	int cu_count = 0;
 	int ntt_algo = VKHEL_NTT_AUTO;
@@ -91,9 +97,24 @@ struct vkhel_vector {
 
 namespace vk {
 
-// Device tables of `t` on `ctx`, uploading them on first use.  nullptr on
-// failure (error recorded).
+// Device tables of `t` on `ctx`, generated on the GPU on first use.
+// nullptr on failure (error recorded).
 const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t);
+
+// Launch arguments of the table generator: the modulus constants and the
+// binary powers w^(2^j) and w^-(2^j), j < log2 n.
+constexpr int kMaxTableLog = 28;
+struct TableGenArgs {
+	uint64_t q, mu, r64, qinv;
+	uint32_t shift, logn;
+	uint64_t wpow[kMaxTableLog], wipow[kMaxTableLog];
+};
+void table_gen_args(uint64_t n, uint64_t q, uint64_t w, TableGenArgs &g);
+// Writes both interleaved {w, Shoup companion} tables of n = 2^g.logn
+// entries (the layout tw_pos describes) on stream `st`.
+hipError_t launch_gen_tables(const TableGenArgs &g, ulonglong2 *fwd,
+		ulonglong2 *inv, hipStream_t st);
+uint64_t inv_mod_2_64(uint64_t q);
 
 // One polynomial transform for the fused kernel: which words, which
 // direction's twiddles, the per-(n, q) constants and the pass split

@@ -65,17 +65,9 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 
 	const uint64_t n = t->n, q = t->q;
 	// {w, packed 63-bit Shoup companion} pairs for both directions, stage
-	// blocks interleaved as tw_pos() lays them out (entry 0 is unused).
-	std::vector<ulonglong2> host(2 * n, make_ulonglong2(0, 0));
-	for (uint64_t m = 1; m < n; m <<= 1) {
-		for (uint64_t i = 0; i < m; i++) {
-			const uint64_t p = tw_pos((uint32_t) m, (uint32_t) i);
-			host[p] = make_ulonglong2(t->roots_of_unity[m + i],
-					shoup63_factor(t->roots_of_unity[m + i], q));
-			host[n + p] = make_ulonglong2(t->inv_roots_of_unity[m + i],
-					shoup63_factor(t->inv_roots_of_unity[m + i], q));
-		}
-	}
+	// blocks interleaved as tw_pos() lays them out, computed on the GPU.
+	TableGenArgs ga;
+	table_gen_args(n, q, t->w, ga);
 	DevTables *d = new DevTables;
 	d->n = n;
 	if (!hip_ok(hipMalloc(&d->alloc, 2 * n * sizeof(ulonglong2)),
@@ -83,8 +75,11 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 		delete d;
 		return nullptr;
 	}
-	if (!hip_ok(hipMemcpy(d->alloc, host.data(), 2 * n * sizeof(ulonglong2),
-					hipMemcpyHostToDevice), "upload tables")) {
+	ulonglong2 *dev = (ulonglong2 *) d->alloc;
+	if (!hip_ok(launch_gen_tables(ga, dev, dev + n, ctx->table_stream),
+				"table generation") ||
+			!hip_ok(hipStreamSynchronize(ctx->table_stream),
+				"table generation sync")) {
 		free_dev_tables(d);
 		return nullptr;
 	}
@@ -100,7 +95,8 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 	// src/vector.c:633: n^-1 by the reference's ext-Euclid.
 	c.ninv = invmod_euclid(n % q, q);
 	c.ninv_p = shoup63_factor(c.ninv, q);
-	c.w1n = mulmod_exact(t->inv_roots_of_unity[n > 1 ? 1 : 0], c.ninv, q);
+	// inv_roots[1] = w^-bitrev(1) = w^-(n/2)
+	c.w1n = mulmod_exact(ga.wipow[ga.logn - 1], c.ninv, q);
 	c.w1n_p = shoup63_factor(c.w1n, q);
 	c.one_p = shoup63_factor(1, q);
 	ctx->tables.emplace(t->id, d);
@@ -143,7 +139,11 @@ struct vkhel_ctx *vkhel_ctx_create() {
 	if (!hip_ok(hipSetDevice(0), "hipSetDevice") ||
 			!hip_ok(hipGetDeviceProperties(&p, 0), "hipGetDeviceProperties") ||
 			!hip_ok(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking),
-				"hipStreamCreate")) {
+				"hipStreamCreate") ||
+			!hip_ok(hipStreamCreateWithFlags(&ctx->table_stream,
+					hipStreamNonBlocking), "hipStreamCreate")) {
+		if (ctx->stream)
+			(void) hipStreamDestroy(ctx->stream);
 		delete ctx;
 		return nullptr;
 	}
@@ -165,17 +165,17 @@ void vkhel_ctx_destroy(struct vkhel_ctx *ctx) {
 	for (auto &kv : ctx->tables)
 		free_dev_tables(kv.second);
 	(void) hipStreamDestroy(ctx->stream);
+	(void) hipStreamDestroy(ctx->table_stream);
 	delete ctx;
 }
 
-// src/ntt_tables.c:17-44, 65-80.  Validation the reference leaves to
-// asserts or undefined behaviour is done up front: n a power of two >= 2
-// (else bitrev indices overflow the arrays), 5 <= q < 2^61 odd (the
+// src/ntt_tables.c:65-80 without the arrays.  Validation the reference
+// leaves to asserts or undefined behaviour is done up front: n a power of
+// two >= 2 (else bitrev indices overflow the arrays), 5 <= q < 2^61 odd (the
 // Barrett validity range, src/numbers.c:8), w < q and gcd(w, q) == 1 (else
 // the ext-Euclid loop divides by zero), n < q (inverse of n).
-struct vkhel_ntt_tables *vkhel_ntt_tables_create(uint64_t n, uint64_t q,
-		uint64_t w) {
-	if (n < 2 || (n & (n - 1)) || n > (1ull << 28))
+static vkhel_ntt_tables *tables_new(uint64_t n, uint64_t q, uint64_t w) {
+	if (n < 2 || (n & (n - 1)) || n > (1ull << kMaxTableLog))
 		return fail(VKHEL_ERR_INVALID,
 				"ntt tables: n=%llu must be a power of two in [2, 2^28]",
 				(unsigned long long) n), nullptr;
@@ -188,20 +188,29 @@ struct vkhel_ntt_tables *vkhel_ntt_tables_create(uint64_t n, uint64_t q,
 				"ntt tables: w=%llu must be < q and invertible mod q",
 				(unsigned long long) w), nullptr;
 	vkhel_ntt_tables *t = (vkhel_ntt_tables *) calloc(1, sizeof *t);
-	uint64_t *mem = (uint64_t *) malloc(4 * n * sizeof(uint64_t));
-	if (!t || !mem) {
-		free(t);
-		free(mem);
+	if (!t)
 		return fail(VKHEL_ERR_NOMEM, "ntt tables: host allocation"), nullptr;
-	}
 	t->n = n;
 	t->q = q;
 	t->w = w;
-	t->roots_of_unity = mem;
-	t->inv_roots_of_unity = mem + n;
-	t->roots_barrett_factors = mem + 2 * n;
-	t->inv_roots_barrett_factors = mem + 3 * n;
+	// w^n by n = 2^k squarings; -1 means w has order exactly 2n.
+	uint64_t p = w;
+	for (uint64_t m = 1; m < n; m <<= 1)
+		p = mulmod_exact(p, p, q);
+	t->root_ok = p == q - 1;
+	std::lock_guard<std::mutex> g(g_ctx_mu);
+	t->id = g_next_tables_id++;
+	return t;
+}
+
+// src/ntt_tables.c:17-44: the host arrays of the reference's struct.
+static bool tables_fill_host(vkhel_ntt_tables *t) {
+	const uint64_t n = t->n, q = t->q, w = t->w;
+	uint64_t *mem = (uint64_t *) malloc(4 * n * sizeof(uint64_t));
+	if (!mem)
+		return fail(VKHEL_ERR_NOMEM, "ntt tables: host allocation"), false;
 	const uint64_t logn = bitlen(n) - 1;
+	uint64_t *roots = mem, *inv_roots = mem + n;
 	// roots[bitrev(i)] = w^i by repeated Barrett products, as upstream.
 	// The inverse table uses w^-i = (w^-1)^i: both it and the reference's
 	// per-entry ext-Euclid yield the unique canonical inverse.
@@ -211,18 +220,33 @@ struct vkhel_ntt_tables *vkhel_ntt_tables_create(uint64_t n, uint64_t q,
 		uint64_t idx = 0;
 		for (uint64_t b = 0, v = i; b < logn; b++, v >>= 1)
 			idx = (idx << 1) | (v & 1);
-		t->roots_of_unity[idx] = r;
-		t->inv_roots_of_unity[idx] = ri;
+		roots[idx] = r;
+		inv_roots[idx] = ri;
 		r = mulmod_barrett(r, w, q);
 		ri = mulmod_exact(ri, winv, q);
 	}
 	for (uint64_t i = 0; i < n; i++) {
-		t->roots_barrett_factors[i] = barrett_factor(t->roots_of_unity[i], q);
-		t->inv_roots_barrett_factors[i] =
-			barrett_factor(t->inv_roots_of_unity[i], q);
+		mem[2 * n + i] = barrett_factor(roots[i], q);
+		mem[3 * n + i] = barrett_factor(inv_roots[i], q);
 	}
-	std::lock_guard<std::mutex> g(g_ctx_mu);
-	t->id = g_next_tables_id++;
+	t->roots_of_unity = roots;
+	t->inv_roots_of_unity = inv_roots;
+	t->roots_barrett_factors = mem + 2 * n;
+	t->inv_roots_barrett_factors = mem + 3 * n;
+	return true;
+}
+
+// Guards the lazy host arrays of tables made by
+// vkhel_ext_tables_create_device.
+static std::mutex g_host_tables_mu;
+
+struct vkhel_ntt_tables *vkhel_ntt_tables_create(uint64_t n, uint64_t q,
+		uint64_t w) {
+	vkhel_ntt_tables *t = tables_new(n, q, w);
+	if (t && !tables_fill_host(t)) {
+		free(t);
+		return nullptr;
+	}
 	return t;
 }
 
@@ -473,6 +497,10 @@ int vkhel_ext_tables_copy(const struct vkhel_ntt_tables *t, uint64_t *roots,
 		uint64_t *inv_roots, uint64_t *roots_bf, uint64_t *inv_roots_bf) {
 	if (!t)
 		return fail(VKHEL_ERR_INVALID, "null tables");
+	std::lock_guard<std::mutex> g(g_host_tables_mu);
+	if (!t->roots_of_unity &&
+			!tables_fill_host(const_cast<vkhel_ntt_tables *>(t)))
+		return vkhel_ext_last_error();
 	const size_t b = t->n * sizeof(uint64_t);
 	if (roots)
 		memcpy(roots, t->roots_of_unity, b);
@@ -489,6 +517,60 @@ int vkhel_ext_tables_upload(struct vkhel_ctx *ctx, struct vkhel_ntt_tables *t) {
 	if (!ctx)
 		return fail(VKHEL_ERR_INVALID, "null ctx");
 	return device_tables(ctx, t) ? VKHEL_OK : vkhel_ext_last_error();
+}
+
+struct vkhel_ntt_tables *vkhel_ext_tables_create_device(struct vkhel_ctx *ctx,
+		uint64_t n, uint64_t q, uint64_t w) {
+	if (!ctx)
+		return fail(VKHEL_ERR_INVALID, "null ctx"), nullptr;
+	vkhel_ntt_tables *t = tables_new(n, q, w);
+	if (!t)
+		return nullptr;
+	if (!t->root_ok) {
+		free(t);
+		return fail(VKHEL_ERR_INVALID,
+				"ntt tables: w=%llu is not a primitive 2n-th root of unity "
+				"mod q (w^n != -1)", (unsigned long long) w), nullptr;
+	}
+	if (!device_tables(ctx, t)) {
+		vkhel_ntt_tables_destroy(t);
+		return nullptr;
+	}
+	return t;
+}
+
+int vkhel_ext_tables_root_ok(const struct vkhel_ntt_tables *t) {
+	if (!t)
+		return fail(VKHEL_ERR_INVALID, "null tables"), -1;
+	return t->root_ok;
+}
+
+int vkhel_ext_tables_read_device(struct vkhel_ctx *ctx,
+		const struct vkhel_ntt_tables *t, uint64_t *roots,
+		uint64_t *roots_shoup, uint64_t *inv_roots, uint64_t *inv_roots_shoup) {
+	if (!ctx || !t)
+		return fail(VKHEL_ERR_INVALID, "null ctx or tables");
+	const DevTables *d = device_tables(ctx, t);
+	if (!d)
+		return vkhel_ext_last_error();
+	const uint64_t n = t->n;
+	std::vector<ulonglong2> h(2 * n);
+	if (!hip_ok(hipMemcpy(h.data(), d->alloc, 2 * n * sizeof(ulonglong2),
+					hipMemcpyDeviceToHost), "read tables"))
+		return vkhel_ext_last_error();
+	for (uint64_t k = 0; k < n; k++) {
+		const uint64_t m = k ? 1ull << (bitlen(k) - 1) : 0;
+		const uint64_t p = k ? tw_pos((uint32_t) m, (uint32_t) (k - m)) : 0;
+		if (roots)
+			roots[k] = h[p].x;
+		if (roots_shoup)
+			roots_shoup[k] = h[p].y;
+		if (inv_roots)
+			inv_roots[k] = h[n + p].x;
+		if (inv_roots_shoup)
+			inv_roots_shoup[k] = h[n + p].y;
+	}
+	return VKHEL_OK;
 }
 
 int vkhel_ext_set_ntt_algo(struct vkhel_ctx *ctx, int algo) {

@@ -63,6 +63,9 @@ SIGNATURES = {
     "vkhel_ext_tables_info": (_int, [_p, _pu64, _pu64, _pu64]),
     "vkhel_ext_tables_copy": (_int, [_p, _pu64, _pu64, _pu64, _pu64]),
     "vkhel_ext_tables_upload": (_int, [_p, _p]),
+    "vkhel_ext_tables_create_device": (_p, [_p, _u64, _u64, _u64]),
+    "vkhel_ext_tables_root_ok": (_int, [_p]),
+    "vkhel_ext_tables_read_device": (_int, [_p, _p, _pu64, _pu64, _pu64, _pu64]),
     "vkhel_ext_set_ntt_algo": (_int, [_p, _int]),
     "vkhel_ext_autotune": (_int, [_p, _u32, _u32, _u32]),
     "vkhel_ext_tile_bits": (_int, [_p, _u32]),

@@ -103,16 +103,44 @@ class Context:
 class NttTables:
     """vkhel_ntt_tables_create(n, q, w) (src/ntt_tables.c:65-80)."""
 
-    def __init__(self, n, q, w):
+    def __init__(self, n, q, w, _handle=None):
         self.n, self.q, self.w = int(n), int(q), int(w)
-        self._h = _L().vkhel_ntt_tables_create(self.n, self.q, self.w)
+        self._h = _handle if _handle is not None else \
+            _L().vkhel_ntt_tables_create(self.n, self.q, self.w)
         if not self._h:
             _lib.check()
             raise _lib.VkhelError(1, "vkhel_ntt_tables_create failed")
 
+    @classmethod
+    def on_device(cls, ctx, n, q, w):
+        """vkhel_ext_tables_create_device: tables generated on the GPU of
+        ctx; w must be a primitive 2n-th root of unity mod q."""
+        h = _L().vkhel_ext_tables_create_device(ctx.handle, int(n), int(q), int(w))
+        if not h:
+            _lib.check()
+            raise _lib.VkhelError(1, "vkhel_ext_tables_create_device failed")
+        return cls(n, q, w, _handle=h)
+
     @property
     def handle(self):
         return self._h
+
+    @property
+    def root_ok(self):
+        """True if w is a primitive 2n-th root of unity mod q."""
+        r = _L().vkhel_ext_tables_root_ok(self._h)
+        if r < 0:
+            _lib.check()
+        return r == 1
+
+    def read_device(self, ctx):
+        """(roots, roots_shoup, inv_roots, inv_roots_shoup) of the device
+        copy on ctx in reference index order; the companions are the packed
+        63-bit ones the kernels read."""
+        out = [np.zeros(self.n, dtype=np.uint64) for _ in range(4)]
+        _lib.check(_L().vkhel_ext_tables_read_device(
+            ctx.handle, self._h, *[_u64p(a) for a in out]))
+        return tuple(out)
 
     def arrays(self):
         """(roots, inv_roots, roots_bf, inv_roots_bf), reference layout."""
