@@ -271,11 +271,11 @@ __device__ __forceinline__ void load_round_tw(ulonglong2 w[7], uint32_t t,
 		const uint32_t M = 1u << (LOGN - 1 - b);
 		const uint32_t STRIDE = M < 4 ? 1 : M >> 2;
 		uint32_t p0 = tw_pos(M, (gbase + l0) >> (lb + 1));
-		// With the field at bits 6-8 the lanes of a wave differ only in
-		// index bits 0-5, which the shift drops: the entries are the same for
-		// the whole wave, and a wave-uniform index lets them load through the
-		// scalar cache.
-		if constexpr (F == 6 && G::THREADS >= 64)
+		// With the field at bit 6 or above, the lanes of a wave differ only
+		// in index bits 0-5 (round_base keeps thread bits 0-5 there), which
+		// the shift drops: the entries are the same for the whole wave, and a
+		// wave-uniform index lets them load through the scalar cache.
+		if constexpr (F >= 6 && G::THREADS >= 64)
 			p0 = __builtin_amdgcn_readfirstlane(p0);
 #pragma unroll
 		for (int gi = 0; gi < (4 >> d); gi++)
