@@ -627,6 +627,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the secondary legs (polymul, uneven, serving)")
+    ap.add_argument("--tile-bits", type=int, default=0,
+                    help="pin the headline size's tile (2^11/2^12/2^13 words, "
+                         "split passes) instead of the tuner's choice, e.g. so "
+                         "a profiled run uses the kernels an unprofiled run "
+                         "picks (kernel tracing serialises dispatches and can "
+                         "change the tuner's ranking)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 copies of 256 MiB (elemgtadd with bound "
                          "2^64-1) to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -651,6 +657,9 @@ def main():
         return
     t_tune = time.perf_counter()
     ctx.autotune(12, 17)
+    if args.tile_bits:
+        ctx.set_tile_bits(args.logn, args.tile_bits)
+        ctx.set_pass_mode(args.logn, 2)
     tuned = {l: ctx.tile_bits(l) for l in range(12, 18)}
     modes = {l: ctx.pass_mode(l) for l in range(12, 18)}
     t_tune = time.perf_counter() - t_tune
