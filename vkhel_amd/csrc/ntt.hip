@@ -189,15 +189,18 @@ static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
 		uint32_t ntasks, uint32_t logn, int tile_bits, bool split, bool inverse,
 		bool small_q, hipStream_t st) {
 	const FusedEntry &e = fused_entry(logn, tile_bits);
-	const bool nr = small_q && !inverse;
+	const bool nr = small_q;
 	if (split && e.tiles > 1) {
 		const dim3 grid(e.tiles, ntasks);
-		hipLaunchKernelGGL(inverse ? e.inv_b : nr ? e.fwd_a_nr : e.fwd_a, grid,
+		hipLaunchKernelGGL(inverse ? (nr ? e.inv_b_nr : e.inv_b)
+				: nr ? e.fwd_a_nr : e.fwd_a, grid,
 				dim3(e.threads), 0, st, uniform, dev_tasks, 0u);
-		hipLaunchKernelGGL(inverse ? e.inv_a : nr ? e.fwd_b_nr : e.fwd_b, grid,
+		hipLaunchKernelGGL(inverse ? (nr ? e.inv_a_nr : e.inv_a)
+				: nr ? e.fwd_b_nr : e.fwd_b, grid,
 				dim3(e.threads), 0, st, uniform, dev_tasks, 1u);
 	} else {
-		hipLaunchKernelGGL(inverse ? e.inv : nr ? e.fwd_nr : e.fwd,
+		hipLaunchKernelGGL(inverse ? (nr ? e.inv_nr : e.inv)
+				: nr ? e.fwd_nr : e.fwd,
 				dim3(ntasks), dim3(e.threads), 0, st, uniform, dev_tasks);
 	}
 	return hip_ok(hipGetLastError(), "ntt fused launch") ? VKHEL_OK

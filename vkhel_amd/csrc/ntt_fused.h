@@ -21,10 +21,10 @@ typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
 struct FusedEntry {
 	FusedKernel fwd, inv;
 	PassKernel fwd_a, fwd_b, inv_b, inv_a;
-	// forward kernels without the butterflies' input reduction (q below
-	// kNoReduceQ)
-	FusedKernel fwd_nr;
-	PassKernel fwd_a_nr, fwd_b_nr;
+	// kernels for q below kNoReduceQ: forward without the butterflies' input
+	// reduction, inverse with the sums reduced once per round
+	FusedKernel fwd_nr, inv_nr;
+	PassKernel fwd_a_nr, fwd_b_nr, inv_b_nr, inv_a_nr;
 	int threads;
 	int tiles;  // tiles per pass (1: one-tile size, passes unused)
 };
@@ -48,6 +48,20 @@ __device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
 	return LazyK{c.two_q, opaque_uniform(0 - c.two_q), opaque_uniform(0 - c.q)};
 }
 
+// 2^64 - 2^h q, h = 1..4, for the inverse's end-of-round reductions.
+struct LazyInvK {
+	uint64_t n[5];
+};
+
+__device__ __forceinline__ LazyInvK lazy_inv_consts(const NttConsts &c) {
+	LazyInvK k;
+	k.n[0] = 0;
+#pragma unroll
+	for (int h = 1; h <= 4; h++)
+		k.n[h] = opaque_uniform(0 - (c.q << h));
+	return k;
+}
+
 __device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
 		const LazyK &k) {
 	const uint64_t xr = csub_n(x, k.two_q, k.n_two_q);
@@ -69,6 +83,17 @@ __device__ __forceinline__ void bf_fwd_nr(uint64_t &x, uint64_t &y,
 __device__ __forceinline__ void bf_inv(uint64_t &x, uint64_t &y, ulonglong2 w,
 		const LazyK &k) {
 	const uint64_t s = csub_n(x + y, k.two_q, k.n_two_q);
+	const uint64_t d = x + k.two_q - y;
+	x = s;
+	y = shoup63(d, w.x, w.y, k.nq);
+}
+
+// First inverse stage: the inverse's inputs are below q (above q the
+// reference's butterfly, nttrevbutterfly.comp:44-48, is not meaningful), so
+// the sum is already below 2q and needs no reduction.
+__device__ __forceinline__ void bf_inv_first(uint64_t &x, uint64_t &y,
+		ulonglong2 w, const LazyK &k) {
+	const uint64_t s = x + y;
 	const uint64_t d = x + k.two_q - y;
 	x = s;
 	y = shoup63(d, w.x, w.y, k.nq);
@@ -110,6 +135,49 @@ __device__ __forceinline__ void bf_inv_last(uint64_t &x, uint64_t &y,
 	const uint64_t d = x + c.two_q - y;
 	x = csub(shoup63(s, c.ninv, c.ninv_p, 0 - c.q), c.q);
 	y = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
+}
+
+// Inverse butterflies for q < kNoReduceQ with the sums left unreduced
+// inside a round.  x and y lie below bx*q and by*q; C = by*q keeps the
+// difference non-negative, and the product input x + C - y stays below
+// (bx + by) q <= 32q < 2^63.  The sum's bound is bx + by.
+__device__ __forceinline__ void bf_inv_nr(uint64_t &x, uint64_t &y,
+		ulonglong2 w, uint64_t C, const LazyK &k) {
+	const uint64_t s = x + y;
+	const uint64_t d = x + C - y;
+	x = s;
+	y = shoup63(d, w.x, w.y, k.nq);
+}
+
+__device__ __forceinline__ void bf_inv_last_nr(uint64_t &x, uint64_t &y,
+		uint64_t C, const NttConsts &c) {
+	const uint64_t s = x + y;
+	const uint64_t d = x + C - y;
+	x = csub(shoup63(s, c.ninv, c.ninv_p, 0 - c.q), c.q);
+	y = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
+}
+
+// Bound, in units of q, of register e of an inverse round after its stages
+// d <= upto (stages whose bit is clear in `active` are absent), when every
+// register entered the round below bin*q: a product output is below 2q, a
+// sum below the sum of its operands' bounds.  Both operands of a sum have
+// the same lower register bits, hence the same history and bound, so a sum
+// doubles it.  All bounds are powers of two.  (Plain loops, not recursion,
+// so that calls with constant arguments fold.)
+__host__ __device__ constexpr int inv_bound(int bin, int active, int upto,
+		int e) {
+	int b = bin;
+	for (int s = 0; s <= upto; s++)
+		if ((active >> s) & 1)
+			b = ((e >> s) & 1) ? 2 : 2 * b;
+	return b;
+}
+
+__host__ __device__ constexpr int log2_exact(int x) {
+	int l = 0;
+	while ((1 << l) < x)
+		l++;
+	return l;
 }
 
 // Tiles hold 2^KT words of one polynomial, 8 per thread.  A pass runs a
@@ -217,9 +285,11 @@ __device__ __forceinline__ void gstore(uint64_t *p, uint32_t i, uint64_t v) {
 	*(gu64 *) ((gchar *) p + (i << 3)) = v;
 }
 
+// Twiddle pair i, addressed like gload: uniform base plus a 32-bit byte
+// offset (the fused kernels serve n <= 2^20, so i << 4 < 2^24).
 __device__ __forceinline__ ulonglong2 gload_tw(const ulonglong2 *p,
 		uint32_t i) {
-	const u64x2 v = ((gcu128 *) p)[i];
+	const u64x2 v = *(gcu128 *) ((gcchar *) p + (i << 4));
 	return make_ulonglong2(v.x, v.y);
 }
 
@@ -423,6 +493,17 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	// ---- butterflies ----
 	if (active) {
 		const LazyK lk = lazy_consts(c);
+		// inverse with deferred sum reductions (q < kNoReduceQ): stages
+		// present in this round, and the bound its inputs enter with (below
+		// q at the transform's first round, below 2q after any other)
+		constexpr bool LAZY_INV = INV && NR;
+		constexpr int ACTIVE = (F >= RD::BLO && F < RD::BHI ? 1 : 0) |
+			(F + 1 >= RD::BLO && F + 1 < RD::BHI ? 2 : 0) |
+			(F + 2 >= RD::BLO && F + 2 < RD::BHI ? 4 : 0);
+		constexpr int BIN = FIRST_ROUND && G::P != 1 ? 1 : 2;
+		constexpr int TOP = LOGN - 1 - G::SHIFT;  // local bit of the last stage
+		constexpr bool HAS_LAST = INV && TOP >= F && TOP <= F + 2 &&
+			TOP >= RD::BLO && TOP < RD::BHI;
 #pragma unroll
 		for (int k = 0; k < 3; k++) {
 			const int d = INV ? k : 2 - k;
@@ -449,21 +530,45 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 				}
 			} else if (INV && b == LOGN - 1) {
 #pragma unroll
-				for (int e = 0; e < 4; e++)
-					bf_inv_last(v[e], v[e + 4], c);
+				for (int e = 0; e < 4; e++) {
+					if constexpr (LAZY_INV)
+						bf_inv_last_nr(v[e], v[e + 4], c.q << log2_exact(
+								inv_bound(BIN, ACTIVE, d - 1, e + 4)), c);
+					else
+						bf_inv_last(v[e], v[e + 4], c);
+				}
 			} else {
 #pragma unroll
 				for (int gi = 0; gi < (4 >> d); gi++) {
 #pragma unroll
 					for (int li = 0; li < (1 << d); li++) {
 						const int e = (gi << (d + 1)) | li;
-						if (INV)
+						if (LAZY_INV)
+							bf_inv_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
+									c.q << log2_exact(inv_bound(BIN, ACTIVE, d - 1,
+											e | (1 << d))), lk);
+						else if (INV && b == 0)
+							bf_inv_first(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+						else if (INV)
 							bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 						else if (NR)
 							bf_fwd_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 						else
 							bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 					}
+				}
+			}
+		}
+		if constexpr (LAZY_INV && !HAS_LAST) {
+			// back below 2q for the next round: halve each bound by
+			// conditional subtractions of (bound / 2) q, ..., 2q
+			const LazyInvK ik = lazy_inv_consts(c);
+#pragma unroll
+			for (int e = 0; e < 8; e++) {
+#pragma unroll
+				for (int h = 4; h >= 1; h--) {
+					if ((1 << h) < inv_bound(BIN, ACTIVE, 2, e))
+						v[e] = csub_n(v[e], c.q << h, ik.n[h]);
 				}
 			}
 		}
@@ -634,8 +739,11 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 		k_ntt_pass<L + kMinFusedLogn, 2, true, TB>,
 		k_ntt_pass<L + kMinFusedLogn, 1, true, TB>,
 		k_ntt_fused<L + kMinFusedLogn, false, TB, true>,
+		k_ntt_fused<L + kMinFusedLogn, true, TB, true>,
 		k_ntt_pass<L + kMinFusedLogn, 1, false, TB, true>,
 		k_ntt_pass<L + kMinFusedLogn, 2, false, TB, true>,
+		k_ntt_pass<L + kMinFusedLogn, 2, true, TB, true>,
+		k_ntt_pass<L + kMinFusedLogn, 1, true, TB, true>,
 		FusedShape<L + kMinFusedLogn, TB>::THREADS,
 		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0)}...};
 }
