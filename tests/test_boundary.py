@@ -109,6 +109,32 @@ def test_table_chain_is_exact_powers(bound):
         assert [int(ir[rev[k]]) for k in range(n)] == [pow(w, -k, q) for k in range(n)]
 
 
+def _small_reduce_consts(q):
+    # restates numbers.h small_reduce_consts
+    sh = max(q.bit_length() - 7, 0)
+    return sh, (1 << (32 + sh)) // q
+
+
+@pytest.mark.parametrize("q", [5, 7, 17, 97, 127, 7681, 12289, 65537,
+                               (1 << 50) - 27, 1125899906826241,
+                               (1 << 57) - 13])
+def test_final_reduction_quotient_is_exact_or_one_less(q):
+    """The forward's last-stage reduction of v < 41q: the estimate
+    floor((v >> sh) * m / 2^32) is floor(v/q) or floor(v/q) - 1, and fits
+    32 bits."""
+    sh, m = _small_reduce_consts(q)
+    assert m < 1 << 32
+    rng = np.random.default_rng(q % 1000)
+    vs = range(41 * q) if q < 200 else \
+        [int(x) for x in rng.integers(0, 41 * q, 20000, dtype=np.uint64)] + \
+        [k * q + d for k in range(41) for d in (0, 1, q - 1) if k * q + d < 41 * q]
+    for v in vs:
+        t = v >> sh
+        assert t < 1 << 13
+        qe = (t * m) >> 32
+        assert v // q - 1 <= qe <= v // q, (v, qe)
+
+
 def test_tables_root_check():
     n = 1 << 8
     q = O.primes_ntt(1 << 40, 2 * n, 1)[0]

@@ -189,6 +189,31 @@ def test_ntt_vs_oracle(ctx, logn, algo):
         ctx.set_ntt_algo(V.NTT_AUTO)
 
 
+@pytest.mark.parametrize("q,max_logn", [(17, 3), (97, 4), (7681, 8),
+                                         (12289, 11), (65537, 15)])
+def test_ntt_small_moduli(ctx, q, max_logn):
+    """Tiny moduli: the reductions' shift and quotient constants at their
+    smallest, every size the modulus admits (2n | q - 1, n < q)."""
+    for logn in range(1, max_logn + 1):
+        n = 1 << logn
+        w = O.root_2n(q, n)
+        t = V.NttTables(n, q, w)
+        for seed in range(3):
+            x = O.fill_uniform(n, q, seed=seed * 100 + logn)
+            if seed == 2:
+                x[:] = q - 1  # every word at the largest residue
+            a = vec(ctx, x)
+            f = ctx.vector(n)
+            a.forward_transform(f, t)
+            want = O.ntt_forward(x, q, w)
+            assert np.array_equal(f.read(), want), (q, logn, seed)
+            f.inverse_transform(f, t)
+            assert np.array_equal(f.read(), x), (q, logn, seed)
+            a.close()
+            f.close()
+        t.close()
+
+
 @pytest.mark.parametrize("bits", [50, 61])
 @pytest.mark.parametrize("logn", [1, 4, 10, 13, 14, 16])
 def test_ntt_noncanonical_inputs_match_reference(ctx, logn, bits):

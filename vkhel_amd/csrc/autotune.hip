@@ -104,7 +104,7 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 		c.ninv_p = shoup63_factor(c.ninv, q);
 		c.w1n = c.ninv;
 		c.w1n_p = c.ninv_p;
-		c.one_p = shoup63_factor(1, q);
+		small_reduce_consts(q, c.red_shift, c.red_m);
 		hipEvent_t e0, e1;
 		if (!hip_ok(hipMemcpyAsync(tw, htw.data(), n * sizeof(ulonglong2),
 						hipMemcpyHostToDevice, s), "autotune upload") ||

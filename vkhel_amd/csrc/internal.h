@@ -34,7 +34,9 @@ struct NttConsts {
 	uint64_t ninv_p;  // floor(ninv * 2^63 / q), the 63-bit Shoup companion
 	uint64_t w1n;     // inv_roots[1] * n^-1 mod q (last inverse stage)
 	uint64_t w1n_p;
-	uint64_t one_p;   // 63-bit Shoup companion of 1 (a reduction mod q)
+	// final reduction of values below 41q (forward, q < kNoReduceQ):
+	// quotient estimate umulhi(v >> red_shift, red_m), see numbers.h
+	uint32_t red_shift, red_m;
 	uint32_t bshift;  // bits - 2
 	uint32_t logn;
 };

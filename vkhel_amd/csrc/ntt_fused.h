@@ -574,10 +574,14 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		}
 		if (!INV && LAST_ROUND && final) {
 			if (NR) {
-				// from below 41q: a Shoup product by 1, then one subtraction
+				// from below 41q: a quotient estimate off by at most one
+				// (small_reduce_consts), then one subtraction
 #pragma unroll
-				for (int e = 0; e < 8; e++)
-					v[e] = csub_n(shoup63(v[e], 1, c.one_p, lk.nq), c.q, lk.nq);
+				for (int e = 0; e < 8; e++) {
+					const uint32_t qe = __umulhi((uint32_t) (v[e] >> c.red_shift),
+							c.red_m);
+					v[e] = csub_n(v[e] + (uint64_t) qe * lk.nq, c.q, lk.nq);
+				}
 			} else {
 #pragma unroll
 				for (int e = 0; e < 8; e++)

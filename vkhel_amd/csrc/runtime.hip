@@ -98,7 +98,7 @@ const DevTables *device_tables(vkhel_ctx *ctx, const vkhel_ntt_tables *t) {
 	// inv_roots[1] = w^-bitrev(1) = w^-(n/2)
 	c.w1n = mulmod_exact(ga.wipow[ga.logn - 1], c.ninv, q);
 	c.w1n_p = shoup63_factor(c.w1n, q);
-	c.one_p = shoup63_factor(1, q);
+	small_reduce_consts(q, c.red_shift, c.red_m);
 	ctx->tables.emplace(t->id, d);
 	return d;
 }
