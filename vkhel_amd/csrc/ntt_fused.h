@@ -48,16 +48,16 @@ __device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
 	return LazyK{c.two_q, opaque_uniform(0 - c.two_q), opaque_uniform(0 - c.q)};
 }
 
-// 2^64 - 2^h q, h = 1..4, for the inverse's end-of-round reductions.
+// 2^64 - 2^h q, h = 2..4, for the inverse's end-of-round reductions.
 struct LazyInvK {
 	uint64_t n[5];
 };
 
 __device__ __forceinline__ LazyInvK lazy_inv_consts(const NttConsts &c) {
 	LazyInvK k;
-	k.n[0] = 0;
+	k.n[0] = k.n[1] = 0;
 #pragma unroll
-	for (int h = 1; h <= 4; h++)
+	for (int h = 2; h <= 4; h++)
 		k.n[h] = opaque_uniform(0 - (c.q << h));
 	return k;
 }
@@ -495,12 +495,12 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		const LazyK lk = lazy_consts(c);
 		// inverse with deferred sum reductions (q < kNoReduceQ): stages
 		// present in this round, and the bound its inputs enter with (below
-		// q at the transform's first round, below 2q after any other)
+		// q at the transform's first round, below 4q after any other)
 		constexpr bool LAZY_INV = INV && NR;
 		constexpr int ACTIVE = (F >= RD::BLO && F < RD::BHI ? 1 : 0) |
 			(F + 1 >= RD::BLO && F + 1 < RD::BHI ? 2 : 0) |
 			(F + 2 >= RD::BLO && F + 2 < RD::BHI ? 4 : 0);
-		constexpr int BIN = FIRST_ROUND && G::P != 1 ? 1 : 2;
+		constexpr int BIN = FIRST_ROUND && G::P != 1 ? 1 : 4;
 		constexpr int TOP = LOGN - 1 - G::SHIFT;  // local bit of the last stage
 		constexpr bool HAS_LAST = INV && TOP >= F && TOP <= F + 2 &&
 			TOP >= RD::BLO && TOP < RD::BHI;
@@ -560,13 +560,16 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 			}
 		}
 		if constexpr (LAZY_INV && !HAS_LAST) {
-			// back below 2q for the next round: halve each bound by
-			// conditional subtractions of (bound / 2) q, ..., 2q
+			// back below 4q for the next round (whose registers each mix
+			// values of every register here): halve each bound by
+			// conditional subtractions of (bound / 2) q, ..., 4q.  Within a
+			// round the bounds then reach 32q (sums) and the products'
+			// inputs stay below 32q < 2^63.
 			const LazyInvK ik = lazy_inv_consts(c);
 #pragma unroll
 			for (int e = 0; e < 8; e++) {
 #pragma unroll
-				for (int h = 4; h >= 1; h--) {
+				for (int h = 4; h >= 2; h--) {
 					if ((1 << h) < inv_bound(BIN, ACTIVE, 2, e))
 						v[e] = csub_n(v[e], c.q << h, ik.n[h]);
 				}
