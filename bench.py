@@ -594,7 +594,8 @@ def traffic_from_profile(logn, npoly, tile_bits):
     from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
     workload (scaled by the copy-kernel calibration recorded beside it), or
     None when no summary of the same tile size exists."""
-    path = os.path.join(REPO, "profiles", f"ntt_fwd_n{1 << logn}_x{npoly}.json")
+    path = os.path.join(REPO, "profiles",
+                        f"ntt_fwd_n{1 << logn}_x{npoly}_tile{tile_bits}.json")
     try:
         with open(path) as f:
             prof = json.load(f)
@@ -724,7 +725,8 @@ def main():
                      "kernel_ms": round(r["fwd_ms"], 4),
                      "inverse_kernel_ms": round(r["inv_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes,
-                     "profile": (f"profiles/ntt_fwd_n{n}_x{npoly}.md" if prof else None),
+                     "profile": (f"profiles/ntt_fwd_n{n}_x{npoly}_tile{tb}.md"
+                                 if prof else None),
                      "profiled_valu_busy": prof.get("valu_busy") if prof and traffic else None},
         "plain_per_call": plain,
         "tile_bits": tuned,
