@@ -182,9 +182,19 @@ int vkhel_batch_add_rns(struct vkhel_batch *, int op,
 		const struct vkhel_vector *a, struct vkhel_vector *res,
 		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs,
 		const uint64_t *params);
-/* Enqueues the batch on the context stream and returns immediately. */
+/* Enqueues the batch on the context stream and returns immediately.  The
+ * batch is built (task arrays uploaded, launch sequence captured) on a
+ * stream of its own, so batches may be built and launched from several
+ * threads while other calls use the same context; one batch object is used
+ * by one thread at a time.  Relaunching or changing a batch waits for its
+ * previous launch. */
 int vkhel_batch_launch(struct vkhel_batch *);
-/* vkhel_batch_launch + wait. */
+/* Completion handle of the last launch: wait blocks until it finished
+ * (VKHEL_OK at once if the batch was not launched), query returns 1 if it
+ * finished, 0 if it is still running, -1 on error. */
+int vkhel_batch_wait(struct vkhel_batch *);
+int vkhel_batch_query(struct vkhel_batch *);
+/* vkhel_batch_launch + vkhel_batch_wait. */
 int vkhel_batch_run(struct vkhel_batch *);
 uint64_t vkhel_batch_num_jobs(const struct vkhel_batch *);
 

@@ -625,3 +625,74 @@ def test_device_created_tables_validate_w_and_transform_alike(ctx):
     assert np.array_equal(outs[1][1], a)
     host.close()
     dev.close()
+
+
+def test_batches_from_several_threads_with_single_calls_between(ctx):
+    """Batches built and launched from different threads, while a third
+    thread issues single calls on the same context: every result is the
+    one each computes alone, and the completion handles report it."""
+    import threading
+    n = 1 << 12
+    q = _prime(12, 50)
+    w = O.root_2n(q, n)
+    t = V.NttTables(n, q, w)
+    rng = np.random.default_rng(5)
+    xs = [[rng.integers(0, q, n, dtype=np.uint64) for _ in range(24)]
+          for _ in range(2)]
+    vs = [[vec(ctx, x) for x in row] for row in xs]
+    outs = [[ctx.vector(n) for _ in row] for row in xs]
+    ys = [rng.integers(0, q, n, dtype=np.uint64) for _ in range(16)]
+    errors = []
+
+    def batch_worker(k):
+        try:
+            for rep in range(4):
+                b = V.Batch(ctx)
+                for v, o in zip(vs[k], outs[k]):
+                    b.add(V.JOB_NTT_FWD, v, None, o, [t])
+                b.launch()
+                b.wait()
+                assert b.done()
+                b.close()
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    def single_worker():
+        try:
+            for y in ys:
+                a = vec(ctx, y)
+                f = ctx.vector(n)
+                a.forward_transform(f, t)
+                assert np.array_equal(f.read(), O.ntt_forward(y, q, w))
+                a.close()
+                f.close()
+        except Exception as e:
+            errors.append(e)
+
+    th = [threading.Thread(target=batch_worker, args=(k,)) for k in range(2)]
+    th.append(threading.Thread(target=single_worker))
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=100)
+    assert not errors, errors
+    for k in range(2):
+        for i in (0, 11, 23):
+            assert np.array_equal(outs[k][i].read(), O.ntt_forward(xs[k][i], q, w))
+    # query before and after an explicit wait; a rebuilt batch waits for the
+    # previous launch before touching its buffers
+    b = V.Batch(ctx)
+    for v, o in zip(vs[0], outs[0]):
+        b.add(V.JOB_NTT_INV, o, None, v, [t])
+    b.launch()
+    b.add(V.JOB_NTT_FWD, vs[1][0], None, outs[1][0], [t])
+    b.launch()
+    b.wait()
+    assert b.done()
+    for i in (0, 5, 23):
+        assert np.array_equal(vs[0][i].read(), xs[0][i])
+    b.close()
+    for row in vs + outs:
+        for v in row:
+            v.close()
+    t.close()

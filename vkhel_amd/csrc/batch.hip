@@ -357,6 +357,14 @@ struct vkhel_batch {
 	size_t scratch_words = 0;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
+	// Private stream the batch uploads its task arrays on and captures its
+	// graph from, so that building a batch never captures work other
+	// threads enqueue on the context stream.
+	hipStream_t build_stream = nullptr;
+	// Recorded on the context stream after every launch: the batch's
+	// completion handle, waited on before its buffers are rebuilt.
+	hipEvent_t done = nullptr;
+	bool launched = false;
 };
 
 namespace vk {
@@ -370,10 +378,28 @@ static void release_graph(vkhel_batch *b) {
 	b->graph = nullptr;
 }
 
+// Waits for the batch's last launch (its buffers and graph are then free
+// to change).
+static int wait_launched(vkhel_batch *b) {
+	if (!b->launched)
+		return VKHEL_OK;
+	if (!hip_ok(hipEventSynchronize(b->done), "batch wait"))
+		return vkhel_ext_last_error();
+	b->launched = false;
+	return VKHEL_OK;
+}
+
 // Turns the job list into device task arrays and a captured graph.
 static int build(vkhel_batch *b) {
+	int rc = wait_launched(b);
+	if (rc != VKHEL_OK)
+		return rc;
 	release_graph(b);
 	vkhel_ctx *ctx = b->ctx;
+	if (!b->build_stream &&
+			!hip_ok(hipStreamCreateWithFlags(&b->build_stream,
+					hipStreamNonBlocking), "batch stream"))
+		return vkhel_ext_last_error();
 	uint64_t scratch_words = 0;
 	for (const Job &j : b->jobs)
 		scratch_words += job_scratch_words(j);
@@ -389,17 +415,17 @@ static int build(vkhel_batch *b) {
 	std::vector<const Job *> jp;
 	for (const Job &j : b->jobs)
 		jp.push_back(&j);
-	int rc = plan_tasks(ctx, jp.data(), jp.size(), b->scratch, b->ts);
+	rc = plan_tasks(ctx, jp.data(), jp.size(), b->scratch, b->ts);
+	hipStream_t s = b->build_stream;
 	if (rc == VKHEL_OK)
-		rc = upload_tasks(b->ts, b->dt, ctx->stream);
+		rc = upload_tasks(b->ts, b->dt, s);
 	if (rc == VKHEL_OK)
-		rc = hip_ok(hipStreamSynchronize(ctx->stream), "sync") ? VKHEL_OK
+		rc = hip_ok(hipStreamSynchronize(s), "sync") ? VKHEL_OK
 			: VKHEL_ERR_DEVICE;
 	if (rc != VKHEL_OK)
 		return rc;
 
 	// Capture the launch sequence once; replays skip per-kernel launch cost.
-	hipStream_t s = ctx->stream;
 	if (!hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal),
 				"begin capture"))
 		return vkhel_ext_last_error();
@@ -435,10 +461,14 @@ struct vkhel_batch *vkhel_batch_create(struct vkhel_ctx *ctx) {
 void vkhel_batch_destroy(struct vkhel_batch *b) {
 	if (!b)
 		return;
-	(void) hipStreamSynchronize(b->ctx->stream);
+	(void) wait_launched(b);
 	release_graph(b);
 	b->dt.release();
 	(void) hipFree(b->scratch);
+	if (b->done)
+		(void) hipEventDestroy(b->done);
+	if (b->build_stream)
+		(void) hipStreamDestroy(b->build_stream);
 	delete b;
 }
 
@@ -492,15 +522,40 @@ int vkhel_batch_launch(struct vkhel_batch *b) {
 		if (rc != VKHEL_OK)
 			return rc;
 	}
-	return hip_ok(hipGraphLaunch(b->exec, b->ctx->stream), "graph launch")
-		? VKHEL_OK : VKHEL_ERR_DEVICE;
+	if (!b->done && !hip_ok(hipEventCreateWithFlags(&b->done,
+				hipEventDisableTiming), "batch event"))
+		return vkhel_ext_last_error();
+	if (!hip_ok(hipGraphLaunch(b->exec, b->ctx->stream), "graph launch") ||
+			!hip_ok(hipEventRecord(b->done, b->ctx->stream), "batch event"))
+		return vkhel_ext_last_error();
+	b->launched = true;
+	return VKHEL_OK;
+}
+
+int vkhel_batch_wait(struct vkhel_batch *b) {
+	if (!b)
+		return fail(VKHEL_ERR_INVALID, "batch_wait: null batch");
+	return wait_launched(b);
+}
+
+int vkhel_batch_query(struct vkhel_batch *b) {
+	if (!b)
+		return fail(VKHEL_ERR_INVALID, "batch_query: null batch"), -1;
+	if (!b->launched)
+		return 1;
+	const hipError_t e = hipEventQuery(b->done);
+	if (e == hipErrorNotReady)
+		return 0;
+	if (!hip_ok(e, "batch query"))
+		return -1;
+	return 1;
 }
 
 int vkhel_batch_run(struct vkhel_batch *b) {
 	const int rc = vkhel_batch_launch(b);
 	if (rc != VKHEL_OK)
 		return rc;
-	return vkhel_ext_ctx_synchronize(b->ctx);
+	return vkhel_batch_wait(b);
 }
 
 }  // extern "C"

@@ -90,6 +90,8 @@ SIGNATURES = {
     "vkhel_batch_add_rns": (_int, [_p, _int, _p, _p, ctypes.POINTER(_p), _u32,
                                    _pu64]),
     "vkhel_batch_launch": (_int, [_p]),
+    "vkhel_batch_wait": (_int, [_p]),
+    "vkhel_batch_query": (_int, [_p]),
     "vkhel_batch_run": (_int, [_p]),
     "vkhel_batch_num_jobs": (_u64, [_p]),
 }

@@ -335,6 +335,17 @@ class Batch:
     def launch(self):
         _lib.check(_L().vkhel_batch_launch(self._h))
 
+    def wait(self):
+        """vkhel_batch_wait: blocks until the last launch finished."""
+        _lib.check(_L().vkhel_batch_wait(self._h))
+
+    def done(self):
+        """vkhel_batch_query: True once the last launch finished."""
+        r = _L().vkhel_batch_query(self._h)
+        if r < 0:
+            _lib.check()
+        return r == 1
+
     def run(self):
         _lib.check(_L().vkhel_batch_run(self._h))
 
