@@ -18,6 +18,8 @@ Also reported, each with its own timing:
   * batched_n14: the same batch at n = 2^14;
   * polymul: 256 pairs x 16 primes < 2^55 at n = 2^15 (NTT a, NTT b,
     pointwise, INTT) as one batch, vs per call;
+  * intt_center: INTT + centered lift into Z_p of the headline batch,
+    fused in the inverse kernel vs as two batches;
   * uneven_batch: 1000 seeded jobs, n = 2^10..2^17, 1..24 primes < 2^60,
     ops NTT / INTT / polymul / add / mul, as one batch and per call;
   * serving: the same 1000 jobs through the serving engine at Poisson
@@ -433,6 +435,48 @@ def uneven_and_serving(V, ctx, ev, quick=False):
     return uneven, serving
 
 
+def intt_center_leg(V, ctx, ev, logn=16, primes=8, per_prime=64, p=65537,
+                    reps=5):
+    """The RNS decryption-side step: INTT of every limb, then the centered
+    lift into Z_p, for the headline batch shape; fused (INTT_CENTER, the
+    lift applied by the inverse kernel as it stores) against the same work
+    as two batches (INTT, then CENTER over the RNS polynomials)."""
+    n = 1 << logn
+    qs = primes_ntt(1 << 50, 2 * n, primes)
+    tabs = [V.NttTables(n, q, root_2n(q, n)) for q in qs]
+    words = primes * n
+    A, R, T = [], [], []
+    for k in range(per_prime):
+        a = ctx.vector(words, False)
+        for i, q in enumerate(qs):
+            a.fill_uniform(q, 11 * k + i, i * n, n)
+        A.append(a)
+        R.append(ctx.vector(words))
+        T.append(ctx.vector(words))
+    fused, inv, center = V.Batch(ctx), V.Batch(ctx), V.Batch(ctx)
+    for a, r, t in zip(A, R, T):
+        fused.add_rns(V.JOB_INTT_CENTER, a, r, tabs, [p])
+        inv.add(V.JOB_NTT_INV, a, None, t, tabs)
+        center.add_rns(V.JOB_CENTER, t, t, tabs, [p])
+    f_ms = timed_launches(ctx, fused, reps, ev)
+    i_ms = timed_launches(ctx, inv, reps, ev)
+    c_ms = timed_launches(ctx, center, reps, ev)
+    same = all(np.array_equal(r.read(), t.read()) for r, t in zip(R[:4], T[:4]))
+    coeffs = per_prime * words
+    res = {"workload": f"INTT + centered lift into Z_{p}, {per_prime} polys x "
+                       f"{primes} primes <2^50, n=2^{logn}",
+           "fused_ms": round(f_ms, 4), "two_step_ms": round(i_ms + c_ms, 4),
+           "fused_gcoeff_s": round(coeffs / f_ms / 1e6, 3),
+           "speedup": round((i_ms + c_ms) / f_ms, 3),
+           "fused_alg_GBps": round(16 * coeffs / f_ms / 1e6, 1),
+           "results_equal": bool(same)}
+    for v in A + R + T:
+        v.close()
+    for b in (fused, inv, center):
+        b.close()
+    return res
+
+
 def pcie_inclusive(V, ctx, logn=16, primes=8, per_prime=64, chunks=8,
                    per_call_sample=32):
     """Host-resident inputs and outputs (the headline batch, n = 2^16,
@@ -691,6 +735,7 @@ def main():
         for b in r14["batches"]:
             b.close()
         extra["polymul"] = polymul_leg(V, ctx, ev)
+        extra["intt_center"] = intt_center_leg(V, ctx, ev)
         extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
         extra["config1"] = config1(V, ctx)
         extra["pcie_inclusive"] = pcie_inclusive(V, ctx)

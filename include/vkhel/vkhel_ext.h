@@ -162,6 +162,10 @@ enum vkhel_job_op {
 	VKHEL_JOB_GTADD = 6,    /* res_i = elemgtadd(a_i, params[2i], params[2i+1]) */
 	VKHEL_JOB_GTSUB = 7,    /* res_i = elemgtsub(a_i, params[3i], params[3i+1],
 	                                             params[3i+2])             */
+	/* fused: res_i = elemmod(INTT(a_i), params[0], q_i), the centered lift
+	 * applied as the inverse transform stores its results (one pass over
+	 * memory less than INTT then CENTER); n from 2^3 to 2^20 */
+	VKHEL_JOB_INTT_CENTER = 8,
 };
 
 struct vkhel_batch;

@@ -696,3 +696,72 @@ def test_batches_from_several_threads_with_single_calls_between(ctx):
         for v in row:
             v.close()
     t.close()
+
+
+@pytest.mark.parametrize("p", [2, 3, 65537, (1 << 40) + 15])
+@pytest.mark.parametrize("logn,tb,mode", [(3, 0, 0), (10, 0, 0), (12, 0, 0),
+                                          (16, 12, 2), (16, 11, 1), (17, 13, 2)])
+def test_fused_intt_center_matches_intt_then_elemmod(ctx, p, logn, tb, mode):
+    """INTT_CENTER (the centered lift applied by the inverse kernel as it
+    stores) equals the oracle's INTT followed by elemmod, per limb, for
+    moduli of both kernel classes, and the two single calls in sequence."""
+    n, L = 1 << logn, 3
+    qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in (50, 55, 61)]
+    ws = [O.root_2n(q, n) for q in qs]
+    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+    if tb:
+        ctx.set_tile_bits(logn, tb)
+        ctx.set_pass_mode(logn, mode)
+    try:
+        x = np.concatenate([O.fill_uniform(n, q, seed=logn + i)
+                            for i, q in enumerate(qs)])
+        a = vec(ctx, x)
+        r = ctx.vector(L * n)
+        b = V.Batch(ctx)
+        b.add_rns(V.JOB_INTT_CENTER, a, r, tabs, [p])
+        b.run()
+        got = r.read()
+        for i, (q, w) in enumerate(zip(qs, ws)):
+            sl = slice(i * n, (i + 1) * n)
+            want = O.elemmod(O.ntt_inverse(x[sl], q, w), p, q)
+            assert np.array_equal(got[sl], want), (i, q)
+            if i == 0:
+                s1, s2 = ctx.vector(n), ctx.vector(n)
+                a.view(0, n).inverse_transform(s1, tabs[0])
+                s1.elemmod(s2, p, q)
+                assert np.array_equal(s2.read(), got[sl])
+                s1.close()
+                s2.close()
+        b.close()
+        a.close()
+        r.close()
+    finally:
+        ctx.set_tile_bits(logn, 0)
+        ctx.set_pass_mode(logn, 0)
+        for t in tabs:
+            t.close()
+
+
+def test_fused_intt_center_rejects_unsupported_sizes(ctx):
+    for n in (2, 4):
+        q = O.primes_ntt(1 << 50, 2 * n, 1)[0]
+        t = V.NttTables(n, q, O.root_2n(q, n))
+        a, r = ctx.vector(n), ctx.vector(n)
+        b = V.Batch(ctx)
+        with pytest.raises(V.VkhelError):
+            b.add_rns(V.JOB_INTT_CENTER, a, r, [t], [3])  # n below 2^3
+        b.close()
+        a.close()
+        r.close()
+        t.close()
+    n = 1 << 8
+    q = O.primes_ntt(1 << 50, 2 * n, 1)[0]
+    t = V.NttTables(n, q, O.root_2n(q, n))
+    a, r = ctx.vector(n), ctx.vector(n)
+    b = V.Batch(ctx)
+    with pytest.raises(V.VkhelError):
+        b.add_rns(V.JOB_INTT_CENTER, a, r, [t], [1])  # p < 2
+    b.close()
+    a.close()
+    r.close()
+    t.close()

@@ -116,7 +116,7 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		const uint64_t *params, Job *out) {
 	if (!a || !res || !tables || nlimbs == 0 || !params)
 		return fail(VKHEL_ERR_INVALID, "rns job: null argument");
-	if (op < VKHEL_JOB_CENTER || op > VKHEL_JOB_GTSUB)
+	if (op < VKHEL_JOB_CENTER || op > VKHEL_JOB_INTT_CENTER)
 		return fail(VKHEL_ERR_INVALID, "rns job: unknown op %d", op);
 	if (a->ctx != ctx || res->ctx != ctx)
 		return fail(VKHEL_ERR_INVALID, "rns job: vector from another context");
@@ -129,10 +129,15 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		return fail(VKHEL_ERR_INVALID,
 				"rns job: vectors need %llu words (%u limbs x n)",
 				(unsigned long long) words, nlimbs);
-	const size_t np = op == VKHEL_JOB_CENTER ? 1 :
+	const bool center = op == VKHEL_JOB_CENTER || op == VKHEL_JOB_INTT_CENTER;
+	const size_t np = center ? 1 :
 		(op == VKHEL_JOB_GTADD ? 2 : 3) * (size_t) nlimbs;
-	if (op == VKHEL_JOB_CENTER && params[0] < 2)
+	if (center && params[0] < 2)
 		return fail(VKHEL_ERR_INVALID, "rns job: center needs p >= 2");
+	if (op == VKHEL_JOB_INTT_CENTER &&
+			!ntt_fused_supported((uint32_t) (bitlen(tables[0]->n) - 1)))
+		return fail(VKHEL_ERR_INVALID,
+				"rns job: intt_center needs n from 2^3 to 2^20");
 	if (op == VKHEL_JOB_GTSUB)
 		for (uint32_t i = 0; i < nlimbs; i++)
 			if (params[3 * i + 2] < 2)
@@ -163,6 +168,7 @@ uint64_t job_transform_coeffs(const Job &j) {
 		return 3 * words;  // two forward transforms and one inverse
 	case VKHEL_JOB_NTT_FWD:
 	case VKHEL_JOB_NTT_INV:
+	case VKHEL_JOB_INTT_CENTER:
 		return words;
 	default:
 		return 0;
@@ -195,7 +201,8 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 		const Job &j = *jobs[ji];
 		const uint64_t n = j.tables[0]->n;
 		// device twiddles only for jobs that transform
-		const bool transforms = j.op <= VKHEL_JOB_POLYMUL;
+		const bool transforms = j.op <= VKHEL_JOB_POLYMUL ||
+			j.op == VKHEL_JOB_INTT_CENTER;
 		for (size_t l = 0; l < j.tables.size(); l++) {
 			const DevTables *t = transforms ? device_tables(ctx, j.tables[l])
 				: nullptr;
@@ -212,6 +219,18 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 			case VKHEL_JOB_NTT_INV:
 				ts.inv.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, true));
 				break;
+			case VKHEL_JOB_INTT_CENTER: {
+				// elemmod applied by the inverse kernel as it stores
+				NttTask inv = make_task(*t, j.a->dev + off, j.res->dev + off, true);
+				const uint64_t pm = j.params[0];
+				inv.c.post_p = pm;
+				if (pm != 2) {
+					inv.c.post_mu = barrett_mu(pm);
+					inv.c.post_shift = (uint32_t) (bitlen(pm) - 2);
+				}
+				ts.inv.push_back(inv);
+				break;
+			}
 			case VKHEL_JOB_POLYMUL: {
 				uint64_t *tmp = sp;
 				sp += n;
@@ -276,7 +295,7 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				break;
 			}
 			}
-			if (j.op >= VKHEL_JOB_MUL)
+			if (j.op >= VKHEL_JOB_MUL && j.op != VKHEL_JOB_INTT_CENTER)
 				ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
 		}
 	}

@@ -37,6 +37,11 @@ struct NttConsts {
 	// final reduction of values below 41q (forward, q < kNoReduceQ):
 	// quotient estimate umulhi(v >> red_shift, red_m), see numbers.h
 	uint32_t red_shift, red_m;
+	// epilogue of an inverse transform (0: none): the result is replaced
+	// by elemmod(result, post_p, q) (src/vector.c:342-386), with the
+	// Barrett constants of post_p
+	uint64_t post_p, post_mu;
+	uint32_t post_shift;
 	uint32_t bshift;  // bits - 2
 	uint32_t logn;
 };

@@ -137,6 +137,24 @@ __device__ __forceinline__ void bf_inv_last(uint64_t &x, uint64_t &y,
 	y = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
 }
 
+// Inverse epilogue: elemmod(x, p, q) of a canonical result x
+// (src/vector.c:342-386): for p = 2 the reference's elemmodbytwo with bound
+// q/2 (elemmodbytwo.comp:19-27), else elemgtsub(x, q/2, q, p)
+// (elemgtsub.comp:49-67), the centered lift of x reduced mod p.
+__device__ __forceinline__ uint64_t post_center(uint64_t x,
+		const NttConsts &c) {
+	const uint64_t half = c.q >> 1;
+	if (c.post_p == 2) {
+		const uint64_t u = x & 1;
+		return x > half ? 1 - u : u;
+	}
+	const uint64_t r = barrett64(x, c.post_p, c.post_mu, c.post_shift);
+	if (x > half)
+		return csub(r + c.post_p - barrett64(c.q, c.post_p, c.post_mu,
+				c.post_shift), c.post_p);
+	return r;
+}
+
 // Inverse butterflies for q < kNoReduceQ with the sums left unreduced
 // inside a round.  x and y lie below bx*q and by*q; C = by*q keeps the
 // difference non-negative, and the product input x + C - y stays below
@@ -573,6 +591,14 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 					if ((1 << h) < inv_bound(BIN, ACTIVE, 2, e))
 						v[e] = csub_n(v[e], c.q << h, ik.n[h]);
 				}
+			}
+		}
+		if constexpr (HAS_LAST) {
+			// the transform's results, canonical: the optional epilogue
+			if (c.post_p) {
+#pragma unroll
+				for (int e = 0; e < 8; e++)
+					v[e] = post_center(v[e], c);
 			}
 		}
 		if (!INV && LAST_ROUND && final) {

@@ -15,6 +15,7 @@ from . import lib as _lib
 NTT_AUTO, NTT_RADIX2, NTT_FUSED = 0, 1, 2
 JOB_NTT_FWD, JOB_NTT_INV, JOB_POLYMUL, JOB_MUL, JOB_ADD = range(5)
 JOB_CENTER, JOB_GTADD, JOB_GTSUB = 5, 6, 7  # RNS elementwise (add_rns)
+JOB_INTT_CENTER = 8  # fused INTT + centered lift (add_rns, params [p])
 
 
 def _L():
@@ -321,8 +322,9 @@ class Batch:
                                         res.handle, arr, len(tabs)))
 
     def add_rns(self, op, a, res, tables, params):
-        """vkhel_batch_add_rns: JOB_CENTER (params [p]), JOB_GTADD
-        ([bound_i, diff_i] per limb) or JOB_GTSUB ([bound_i, diff_i, mod_i])."""
+        """vkhel_batch_add_rns: JOB_CENTER or JOB_INTT_CENTER (params [p]),
+        JOB_GTADD ([bound_i, diff_i] per limb) or JOB_GTSUB
+        ([bound_i, diff_i, mod_i])."""
         tabs = list(tables)
         arr = (ctypes.c_void_p * len(tabs))(*[t.handle for t in tabs])
         prm = np.ascontiguousarray(params, dtype=np.uint64)
