@@ -675,6 +675,7 @@ def test_batches_from_several_threads_with_single_calls_between(ctx):
         x.start()
     for x in th:
         x.join(timeout=100)
+    assert not any(x.is_alive() for x in th), "a worker thread did not finish"
     assert not errors, errors
     for k in range(2):
         for i in (0, 11, 23):
