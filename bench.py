@@ -183,7 +183,7 @@ def pct(xs, p):
 # ---------------------------------------------------------------- legs
 
 def batched_ntt(V, ctx, logn, primes, per_prime, steps, warmup, ev):
-    """Config 2: forward + inverse of primes*per_prime polynomials."""
+    """Batched forward + inverse of primes*per_prime polynomials."""
     n = 1 << logn
     qs = primes_ntt(1 << 50, 2 * n, primes)
     tabs = [V.NttTables(n, q, root_2n(q, n)) for q in qs]
@@ -244,7 +244,8 @@ def plain_per_call(V, ctx, r, k):
 
 
 def polymul_leg(V, ctx, ev, pairs_per_prime=256, primes=16, logn=15, reps=5):
-    """Config 3: a*b mod (X^n + 1, q_i) for 256 pairs x 16 primes < 2^55."""
+    """Polynomial products a*b mod (X^n + 1, q_i), 256 pairs x 16 primes
+    < 2^55 at n = 2^15."""
     n = 1 << logn
     qs = primes_ntt(1 << 55, 2 * n, primes)
     tabs = [V.NttTables(n, q, root_2n(q, n)) for q in qs]
@@ -291,8 +292,8 @@ def polymul_leg(V, ctx, ev, pairs_per_prime=256, primes=16, logn=15, reps=5):
 
 
 def make_uneven_jobs(V, ctx, njobs=1000, seed=20261202):
-    """Config 4: n = 2^10..2^17, L = 1..24 primes < 2^60 (q = 1 mod 2^18),
-    op uniform over NTT, INTT, polymul, add, mul."""
+    """Uneven batch: n = 2^10..2^17, L = 1..24 primes < 2^60 (q = 1 mod
+    2^18), op uniform over NTT, INTT, polymul, add, mul."""
     rng = np.random.default_rng(seed)
     qs = primes_ntt(1 << 60, 1 << 18, 24)
     tables = {}
@@ -608,9 +609,9 @@ def cpu_baseline(logn, qs, ws, target_cpu_s=16.0, min_runs=5):
             "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
-def config1(V, ctx):
-    """Config 1: one N=4096 polymul, q the largest 60-bit prime = 1 mod
-    8192; CPU oracle (one thread) vs the per-call GPU API."""
+def plain_polymul_n4096(V, ctx):
+    """One n = 4096 polymul, q the largest 60-bit prime = 1 mod 8192: CPU
+    oracle (one thread) vs the per-call GPU API, bit-exact check."""
     from oracle import oracle as O
     n = 4096
     q = primes_ntt(1 << 60, 2 * n, 1)[0]
@@ -737,7 +738,7 @@ def main():
         extra["polymul"] = polymul_leg(V, ctx, ev)
         extra["intt_center"] = intt_center_leg(V, ctx, ev)
         extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
-        extra["config1"] = config1(V, ctx)
+        extra["plain_polymul_n4096"] = plain_polymul_n4096(V, ctx)
         extra["pcie_inclusive"] = pcie_inclusive(V, ctx)
 
     cpu = None if args.no_cpu else cpu_baseline(args.logn, qs, ws)
