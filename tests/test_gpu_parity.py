@@ -766,3 +766,38 @@ def test_fused_intt_center_rejects_unsupported_sizes(ctx):
     a.close()
     r.close()
     t.close()
+
+
+@pytest.mark.parametrize("bits", [50, 61])
+def test_ntt_above_fused_sizes(ctx, bits):
+    """n = 2^21, beyond the fused kernels' 2^20: the per-stage path, per call
+    and in a batch (including a polymul, whose pointwise product then runs
+    as its own kernel), against the oracle."""
+    logn = 21
+    n = 1 << logn
+    q = _prime(logn, bits)
+    w = O.root_2n(q, n)
+    t = V.NttTables.on_device(ctx, n, q, w)
+    x = O.fill_uniform(n, q, seed=bits)
+    y = O.fill_uniform(n, q, seed=bits + 1)
+    want = O.ntt_forward(x, q, w)
+    a = vec(ctx, x)
+    f = ctx.vector(n)
+    a.forward_transform(f, t)
+    assert np.array_equal(f.read(), want)
+    f.inverse_transform(f, t)
+    assert np.array_equal(f.read(), x)
+    # batch: forward job and a polymul whose product is checked through the
+    # convolution theorem on the oracle's transforms
+    bv, r, p = vec(ctx, y), ctx.vector(n), ctx.vector(n)
+    batch = V.Batch(ctx)
+    batch.add(V.JOB_NTT_FWD, a, None, r, [t])
+    batch.add(V.JOB_POLYMUL, a, bv, p, [t])
+    batch.run()
+    assert np.array_equal(r.read(), want)
+    prod = O.ntt_inverse(O.elemmul(want, O.ntt_forward(y, q, w), q), q, w)
+    assert np.array_equal(p.read(), prod)
+    batch.close()
+    for v in (a, f, bv, r, p):
+        v.close()
+    t.close()
