@@ -459,7 +459,9 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	constexpr bool LAST_ROUND = R == G::ROUNDS - 1;
 	constexpr bool LOCAL = round_wave_local<G, INV, R>();
 	const uint32_t l0 = round_base<F>(t);
-	const bool active = t < (uint32_t) G::THREADS;
+	// workgroups hold exactly G::THREADS threads, or 64 when a one-tile
+	// size needs fewer
+	const bool active = G::THREADS >= 64 || t < (uint32_t) G::THREADS;
 
 	// ---- operands ----
 	// the first loads of an inverse transform may form a pointwise product
