@@ -653,7 +653,7 @@ def traffic_from_profile(logn, npoly, tile_bits):
 
 def fwd_kernel_names(logn, tile_bits, pass_mode, small_q):
     """The dispatches of one forward transform at this size and tuning
-    (moduli below 2^57 use the kernels without butterfly input reduction)."""
+    (moduli below 2^56 use the small-modulus kernels)."""
     nr = ", true" if small_q else ""
     if logn <= tile_bits or pass_mode == 1:
         return [f"k_ntt_fused<{logn}, false, {tile_bits}{nr}>"]
@@ -767,7 +767,7 @@ def main():
                      "traffic": traffic,
                      "kernel": "forward NTT of the batch: " + " + ".join(
                          fwd_kernel_names(args.logn, tb, ctx.pass_mode(args.logn),
-                                          max(qs) < (1 << 57))),
+                                          max(qs) < (1 << 56))),
                      "kernel_ms": round(r["fwd_ms"], 4),
                      "inverse_kernel_ms": round(r["inv_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes,

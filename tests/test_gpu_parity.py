@@ -166,9 +166,10 @@ def test_ntt_vs_oracle(ctx, logn, algo):
     ctx.set_ntt_algo(algo)
     try:
         # 61 bits: the largest moduli accepted, where the lazy bound 4q
-        # comes closest to 2^63; 57 bits: the largest moduli of the kernels
-        # without per-stage reductions, where their bounds come closest
-        for bits in (50, 57, 60, 61):
+        # comes closest to 2^63; 56 bits: the largest moduli of the
+        # small-modulus kernels, where their bounds come closest; 57 bits:
+        # the smallest of the others
+        for bits in (50, 56, 57, 60, 61):
             q = _prime(logn, bits)
             w = O.root_2n(q, n)
             t = V.NttTables(n, q, w)

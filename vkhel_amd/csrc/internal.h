@@ -34,7 +34,7 @@ struct NttConsts {
 	uint64_t ninv_p;  // floor(ninv * 2^63 / q), the 63-bit Shoup companion
 	uint64_t w1n;     // inv_roots[1] * n^-1 mod q (last inverse stage)
 	uint64_t w1n_p;
-	// final reduction of values below 41q (forward, q < kNoReduceQ):
+	// final reduction of values below 81q (forward, q < kNoReduceQ):
 	// quotient estimate umulhi(v >> red_shift, red_m), see numbers.h
 	uint32_t red_shift, red_m;
 	// epilogue of an inverse transform (0: none): the result is replaced
@@ -136,11 +136,12 @@ __host__ __device__ __forceinline__ uint32_t tw_pos(uint32_t m, uint32_t i) {
 	return m < 4 ? m + i : m + (i & 3) * (m >> 2) + (i >> 2);
 }
 
-// Forward transforms over moduli below this bound run without the
-// butterflies' input reduction: values entering stage s >= 1 stay below
-// (2s + 1) q, i.e. below 41 q < 2^63 for log2 n <= 20, which the 63-bit
-// Shoup product accepts.
-constexpr uint64_t kNoReduceQ = 1ull << 57;
+// Transforms over moduli below this bound use the small-modulus kernels:
+// forward butterflies without input reduction and with products below 4q,
+// so values entering stage s >= 1 stay below (4s + 1) q, below 81q < 2^63
+// after the last stage of n = 2^20; the inverse defers its sum reductions
+// (bounds below 64q).  The 63-bit Shoup product accepts all of these.
+constexpr uint64_t kNoReduceQ = 1ull << 56;
 
 struct NttTask {
 	const uint64_t *in;

@@ -71,13 +71,18 @@ __device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b,
 // 2^64, so Q = 2*y1*p1 + floor(S / 2^31) exactly, from one high multiply,
 // three multiply-adds and one shift, with no carry chain.  R is formed modulo 2^64 as
 // y*w + Q*nq with nq = 2^64 - q.
-__device__ __forceinline__ uint64_t shoup63(uint64_t y, uint64_t w,
+// LOW = false drops the low partial product floor(y0*p0 / 2^32) from S:
+// that moves S / 2^31 down by less than 2, so the quotient is Q, Q - 1 or
+// Q - 2 and R lies in [0, 4q), for one multiply fewer (shoup63_lazy).
+template <bool LOW>
+__device__ __forceinline__ uint64_t shoup63_impl(uint64_t y, uint64_t w,
 		uint64_t wpk, uint64_t nq) {
 	const uint32_t y0 = (uint32_t) y, y1 = (uint32_t) (y >> 32);
 	const uint32_t p0 = (uint32_t) wpk, p1x2 = (uint32_t) (wpk >> 32);
 	const uint32_t p1 = p1x2 >> 1;
 	// (the opaque() points keep each sum a single multiply-add)
-	const uint64_t s0 = opaque((uint64_t) y0 * p1 + __umulhi(y0, p0));
+	const uint64_t s0 = opaque((uint64_t) y0 * p1 +
+			(LOW ? __umulhi(y0, p0) : 0u));
 	const uint64_t s = (uint64_t) y1 * p0 + s0;
 	const uint64_t qt = (uint64_t) y1 * p1x2 + (s >> 31);
 	const uint32_t w0 = (uint32_t) w, w1 = (uint32_t) (w >> 32);
@@ -92,6 +97,17 @@ __device__ __forceinline__ uint64_t shoup63(uint64_t y, uint64_t w,
 	c = (uint64_t) q1 * n0 + c;
 	const uint32_t rh = opaque32((uint32_t) (r >> 32) + (uint32_t) c);
 	return ((uint64_t) rh << 32) | (uint32_t) r;
+}
+
+__device__ __forceinline__ uint64_t shoup63(uint64_t y, uint64_t w,
+		uint64_t wpk, uint64_t nq) {
+	return shoup63_impl<true>(y, w, wpk, nq);
+}
+
+// As shoup63 with R in [0, 4q) instead of [0, 2q).
+__device__ __forceinline__ uint64_t shoup63_lazy(uint64_t y, uint64_t w,
+		uint64_t wpk, uint64_t nq) {
+	return shoup63_impl<false>(y, w, wpk, nq);
 }
 
 // Reference 64-bit Barrett (elemmul.comp:36-47, elemgtsub.comp:36-47):

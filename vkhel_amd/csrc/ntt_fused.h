@@ -42,22 +42,24 @@ constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
 // and its packed 63-bit Shoup companion (shoup63_factor).
 struct LazyK {
 	uint64_t two_q, n_two_q, nq;  // 2q, 2^64 - 2q, 2^64 - q
+	uint64_t four_q;
 };
 
 __device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
-	return LazyK{c.two_q, opaque_uniform(0 - c.two_q), opaque_uniform(0 - c.q)};
+	return LazyK{c.two_q, opaque_uniform(0 - c.two_q), opaque_uniform(0 - c.q),
+		c.two_q << 1};
 }
 
-// 2^64 - 2^h q, h = 2..4, for the inverse's end-of-round reductions.
+// 2^64 - 2^h q, h = 3..5, for the inverse's end-of-round reductions.
 struct LazyInvK {
-	uint64_t n[5];
+	uint64_t n[6];
 };
 
 __device__ __forceinline__ LazyInvK lazy_inv_consts(const NttConsts &c) {
 	LazyInvK k;
-	k.n[0] = k.n[1] = 0;
+	k.n[0] = k.n[1] = k.n[2] = 0;
 #pragma unroll
-	for (int h = 2; h <= 4; h++)
+	for (int h = 3; h <= 5; h++)
 		k.n[h] = opaque_uniform(0 - (c.q << h));
 	return k;
 }
@@ -70,14 +72,15 @@ __device__ __forceinline__ void bf_fwd(uint64_t &x, uint64_t &y, ulonglong2 w,
 	y = xr + k.two_q - t;
 }
 
-// Forward butterfly without the input reduction (q < kNoReduceQ): each
-// stage grows the bound by less than 2q.
+// Forward butterfly without the input reduction (q < kNoReduceQ), its
+// product from shoup63_lazy (below 4q): each stage grows the bound by less
+// than 4q.
 __device__ __forceinline__ void bf_fwd_nr(uint64_t &x, uint64_t &y,
 		ulonglong2 w, const LazyK &k) {
 	const uint64_t xr = x;
-	const uint64_t t = shoup63(y, w.x, w.y, k.nq);
+	const uint64_t t = shoup63_lazy(y, w.x, w.y, k.nq);
 	x = xr + t;
-	y = xr + k.two_q - t;
+	y = xr + k.four_q - t;
 }
 
 __device__ __forceinline__ void bf_inv(uint64_t &x, uint64_t &y, ulonglong2 w,
@@ -100,14 +103,20 @@ __device__ __forceinline__ void bf_inv_first(uint64_t &x, uint64_t &y,
 }
 
 // First forward stage when both inputs are already below q: the lazy
-// butterfly without its input reduction.  The reference's first stage
-// gives the canonical forms of these same residues.
+// butterfly without its input reduction (with NR, the one of bf_fwd_nr).
+// The reference's first stage gives the canonical forms of these same
+// residues.
+template <bool NR>
 __device__ __forceinline__ void bf_fwd_first_canon(uint64_t &x, uint64_t &y,
 		ulonglong2 w, const LazyK &k) {
-	const uint64_t t = shoup63(y, w.x, w.y, k.nq);
-	const uint64_t xr = x;
-	x = xr + t;
-	y = xr + k.two_q - t;
+	if constexpr (NR) {
+		bf_fwd_nr(x, y, w, k);
+	} else {
+		const uint64_t t = shoup63(y, w.x, w.y, k.nq);
+		const uint64_t xr = x;
+		x = xr + t;
+		y = xr + k.two_q - t;
+	}
 }
 
 // First forward stage on arbitrary 64-bit words, exactly as the reference:
@@ -158,13 +167,14 @@ __device__ __forceinline__ uint64_t post_center(uint64_t x,
 // Inverse butterflies for q < kNoReduceQ with the sums left unreduced
 // inside a round.  x and y lie below bx*q and by*q; C = by*q keeps the
 // difference non-negative, and the product input x + C - y stays below
-// (bx + by) q <= 32q < 2^63.  The sum's bound is bx + by.
+// (bx + by) q <= 64q < 2^63.  The sum's bound is bx + by, the product's
+// (shoup63_lazy) 4q.
 __device__ __forceinline__ void bf_inv_nr(uint64_t &x, uint64_t &y,
 		ulonglong2 w, uint64_t C, const LazyK &k) {
 	const uint64_t s = x + y;
 	const uint64_t d = x + C - y;
 	x = s;
-	y = shoup63(d, w.x, w.y, k.nq);
+	y = shoup63_lazy(d, w.x, w.y, k.nq);
 }
 
 __device__ __forceinline__ void bf_inv_last_nr(uint64_t &x, uint64_t &y,
@@ -177,7 +187,7 @@ __device__ __forceinline__ void bf_inv_last_nr(uint64_t &x, uint64_t &y,
 
 // Bound, in units of q, of register e of an inverse round after its stages
 // d <= upto (stages whose bit is clear in `active` are absent), when every
-// register entered the round below bin*q: a product output is below 2q, a
+// register entered the round below bin*q: a product output is below 4q, a
 // sum below the sum of its operands' bounds.  Both operands of a sum have
 // the same lower register bits, hence the same history and bound, so a sum
 // doubles it.  All bounds are powers of two.  (Plain loops, not recursion,
@@ -187,7 +197,7 @@ __host__ __device__ constexpr int inv_bound(int bin, int active, int upto,
 	int b = bin;
 	for (int s = 0; s <= upto; s++)
 		if ((active >> s) & 1)
-			b = ((e >> s) & 1) ? 2 : 2 * b;
+			b = ((e >> s) & 1) ? 4 : 2 * b;
 	return b;
 }
 
@@ -515,12 +525,12 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		const LazyK lk = lazy_consts(c);
 		// inverse with deferred sum reductions (q < kNoReduceQ): stages
 		// present in this round, and the bound its inputs enter with (below
-		// q at the transform's first round, below 4q after any other)
+		// q at the transform's first round, below 8q after any other)
 		constexpr bool LAZY_INV = INV && NR;
 		constexpr int ACTIVE = (F >= RD::BLO && F < RD::BHI ? 1 : 0) |
 			(F + 1 >= RD::BLO && F + 1 < RD::BHI ? 2 : 0) |
 			(F + 2 >= RD::BLO && F + 2 < RD::BHI ? 4 : 0);
-		constexpr int BIN = FIRST_ROUND && G::P != 1 ? 1 : 4;
+		constexpr int BIN = FIRST_ROUND && G::P != 1 ? 1 : 8;
 		constexpr int TOP = LOGN - 1 - G::SHIFT;  // local bit of the last stage
 		constexpr bool HAS_LAST = INV && TOP >= F && TOP <= F + 2 &&
 			TOP >= RD::BLO && TOP < RD::BHI;
@@ -542,7 +552,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 				if (__all(canon)) {
 #pragma unroll
 					for (int e = 0; e < 4; e++)
-						bf_fwd_first_canon(v[e], v[e + 4], w[0], lk);
+						bf_fwd_first_canon<NR>(v[e], v[e + 4], w[0], lk);
 				} else {
 #pragma unroll
 					for (int e = 0; e < 4; e++)
@@ -580,16 +590,16 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 			}
 		}
 		if constexpr (LAZY_INV && !HAS_LAST) {
-			// back below 4q for the next round (whose registers each mix
+			// back below 8q for the next round (whose registers each mix
 			// values of every register here): halve each bound by
-			// conditional subtractions of (bound / 2) q, ..., 4q.  Within a
-			// round the bounds then reach 32q (sums) and the products'
-			// inputs stay below 32q < 2^63.
+			// conditional subtractions of (bound / 2) q, ..., 8q.  Within a
+			// round the bounds then reach 64q (sums) and the products'
+			// inputs stay below 64q < 2^63.
 			const LazyInvK ik = lazy_inv_consts(c);
 #pragma unroll
 			for (int e = 0; e < 8; e++) {
 #pragma unroll
-				for (int h = 4; h >= 2; h--) {
+				for (int h = 5; h >= 3; h--) {
 					if ((1 << h) < inv_bound(BIN, ACTIVE, 2, e))
 						v[e] = csub_n(v[e], c.q << h, ik.n[h]);
 				}
@@ -605,7 +615,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		}
 		if (!INV && LAST_ROUND && final) {
 			if (NR) {
-				// from below 41q: a quotient estimate off by at most one
+				// from below 81q: a quotient estimate off by at most one
 				// (small_reduce_consts), then one subtraction
 #pragma unroll
 				for (int e = 0; e < 8; e++) {

@@ -81,9 +81,9 @@ inline uint64_t invmod_euclid(uint64_t input, uint64_t q) {
 }
 
 // 2^64 mod q.
-// Constants of a one-multiply reduction of v < 41q to [0, q): with
+// Constants of a one-multiply reduction of v < 81q to [0, q): with
 // L = bitlen(q), shift = max(L - 7, 0) and m = floor(2^(32 + shift) / q),
-// T = v >> shift is below 41 * 2^7 < 2^13 and Q = floor(T * m / 2^32)
+// T = v >> shift is below 81 * 2^7 < 2^14 and Q = floor(T * m / 2^32)
 // satisfies floor(v/q) - 1 <= Q <= floor(v/q): Q <= T 2^shift / q <= v/q,
 // and T m / 2^32 > v/q - 2^shift/q - T/2^32 > v/q - 1 (2^shift/q <= 1/5).
 // So v - Q q < 2q and one conditional subtraction finishes; m < 2^32.
