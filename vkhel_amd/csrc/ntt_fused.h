@@ -139,11 +139,11 @@ __device__ __forceinline__ void bf_fwd_first(uint64_t &x, uint64_t &y,
 // n^-1 (src/vector.c:633-638) folded in; canonical outputs.  ninv_p and
 // w1n_p are 63-bit Shoup companions.
 __device__ __forceinline__ void bf_inv_last(uint64_t &x, uint64_t &y,
-		const NttConsts &c) {
-	const uint64_t s = csub(x + y, c.two_q);
-	const uint64_t d = x + c.two_q - y;
-	x = csub(shoup63(s, c.ninv, c.ninv_p, 0 - c.q), c.q);
-	y = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
+		const NttConsts &c, const LazyK &k) {
+	const uint64_t s = csub_n(x + y, k.two_q, k.n_two_q);
+	const uint64_t d = x + k.two_q - y;
+	x = csub_n(shoup63(s, c.ninv, c.ninv_p, k.nq), c.q, k.nq);
+	y = csub_n(shoup63(d, c.w1n, c.w1n_p, k.nq), c.q, k.nq);
 }
 
 // Inverse epilogue: elemmod(x, p, q) of a canonical result x
@@ -178,11 +178,11 @@ __device__ __forceinline__ void bf_inv_nr(uint64_t &x, uint64_t &y,
 }
 
 __device__ __forceinline__ void bf_inv_last_nr(uint64_t &x, uint64_t &y,
-		uint64_t C, const NttConsts &c) {
+		uint64_t C, const NttConsts &c, const LazyK &k) {
 	const uint64_t s = x + y;
 	const uint64_t d = x + C - y;
-	x = csub(shoup63(s, c.ninv, c.ninv_p, 0 - c.q), c.q);
-	y = csub(shoup63(d, c.w1n, c.w1n_p, 0 - c.q), c.q);
+	x = csub_n(shoup63(s, c.ninv, c.ninv_p, k.nq), c.q, k.nq);
+	y = csub_n(shoup63(d, c.w1n, c.w1n_p, k.nq), c.q, k.nq);
 }
 
 // Bound, in units of q, of register e of an inverse round after its stages
@@ -563,9 +563,9 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 				for (int e = 0; e < 4; e++) {
 					if constexpr (LAZY_INV)
 						bf_inv_last_nr(v[e], v[e + 4], c.q << log2_exact(
-								inv_bound(BIN, ACTIVE, d - 1, e + 4)), c);
+								inv_bound(BIN, ACTIVE, d - 1, e + 4)), c, lk);
 					else
-						bf_inv_last(v[e], v[e + 4], c);
+						bf_inv_last(v[e], v[e + 4], c, lk);
 				}
 			} else {
 #pragma unroll
