@@ -166,6 +166,14 @@ enum vkhel_job_op {
 	 * applied as the inverse transform stores its results (one pass over
 	 * memory less than INTT then CENTER); n from 2^3 to 2^20 */
 	VKHEL_JOB_INTT_CENTER = 8,
+	/* RNS rescaling by the last modulus (nlimbs >= 2, params unused):
+	 * res_i = (a_i - [a_last]) * q_last^-1 mod q_i for i < nlimbs - 1, where
+	 * [a_last] is the centered lift of the last limb (a_last - q_last when
+	 * a_last > q_last / 2); res holds nlimbs - 1 limbs and is the rounded
+	 * quotient A / q_last of the integer A the limbs represent.  Inputs
+	 * canonical; res must not alias a.  Runs after every transform of the
+	 * batch, so it may consume an INTT of the same batch. */
+	VKHEL_JOB_RESCALE = 9,
 };
 
 struct vkhel_batch;

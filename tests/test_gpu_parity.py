@@ -802,3 +802,68 @@ def test_ntt_above_fused_sizes(ctx, bits):
     for v in (a, f, bv, r, p):
         v.close()
     t.close()
+
+
+def _rescale_oracle(limbs, qs):
+    """Exact big-integer statement of RNS rescaling by the last modulus:
+    A from the residues (CRT), t the centered residue of A mod q_last,
+    A' = (A - t) / q_last exactly; the residues of A' mod the other q."""
+    Q = 1
+    for q in qs:
+        Q *= q
+    ql = qs[-1]
+    n = len(limbs[0])
+    out = [np.zeros(n, dtype=np.uint64) for _ in qs[:-1]]
+    coef = [(Q // q) * pow(Q // q, -1, q) for q in qs]
+    for j in range(n):
+        A = sum(int(limbs[i][j]) * coef[i] for i in range(len(qs))) % Q
+        r = A % ql
+        t = r - ql if r > ql // 2 else r
+        Ap = (A - t) // ql
+        assert (A - t) % ql == 0
+        assert Ap == (2 * A + ql) // (2 * ql)  # the rounded quotient A / q_last
+        for i, q in enumerate(qs[:-1]):
+            out[i][j] = Ap % q
+    return out
+
+
+@pytest.mark.parametrize("L", [2, 3, 4])
+def test_rns_rescale_matches_big_integer_division(ctx, L):
+    n = 1 << 10
+    qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in (50, 61, 55, 40)[:L]]
+    ws = [O.root_2n(q, n) for q in qs]
+    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+    rng = np.random.default_rng(L)
+    limbs = [rng.integers(0, q, n, dtype=np.uint64) for q in qs]
+    # the dropped limb's centered-lift boundary values
+    limbs[-1][:4] = [0, qs[-1] // 2, qs[-1] // 2 + 1, qs[-1] - 1]
+    a = vec(ctx, np.concatenate(limbs))
+    r = ctx.vector((L - 1) * n)
+    b = V.Batch(ctx)
+    b.add_rns(V.JOB_RESCALE, a, r, tabs)
+    b.run()
+    want = _rescale_oracle(limbs, qs)
+    got = r.read()
+    for i in range(L - 1):
+        assert np.array_equal(got[i * n:(i + 1) * n], want[i]), i
+    # INTT then rescale in one batch: the rescale runs after the transforms
+    x = ctx.vector(L * n)
+    spec = vec(ctx, np.concatenate([O.ntt_forward(l, q, w)
+                                    for l, q, w in zip(limbs, qs, ws)]))
+    r2 = ctx.vector((L - 1) * n)
+    b2 = V.Batch(ctx)
+    b2.add(V.JOB_NTT_INV, spec, None, x, tabs)
+    b2.add_rns(V.JOB_RESCALE, x, r2, tabs)
+    b2.run()
+    assert np.array_equal(r2.read(), got)
+    # rejected: one limb, result aliasing the input
+    with pytest.raises(V.VkhelError):
+        b2.add_rns(V.JOB_RESCALE, a, r, tabs[:1])
+    with pytest.raises(V.VkhelError):
+        b2.add_rns(V.JOB_RESCALE, a, a, tabs)
+    for o in (b, b2):
+        o.close()
+    for v in (a, r, x, spec, r2):
+        v.close()
+    for t in tabs:
+        t.close()

@@ -41,6 +41,13 @@ __device__ __forceinline__ uint64_t point_op(const PointTask &t, uint64_t x,
 	}
 	case kPtGtAdd:  // elemgtadd.comp:20-30
 		return x > t.bound ? x + t.diff : x;
+	case kPtRescale: {
+		// (a - [b]) / q_last mod q: a canonical mod q, b mod q_last
+		const uint64_t bm = barrett64(y, t.q, t.mu, t.shift);
+		const uint64_t c = y > t.bound ? sub_mod(bm, t.diff, t.q) : bm;
+		const uint64_t d = sub_mod(barrett64(x, t.q, t.mu, t.shift), c, t.q);
+		return csub(shoup63(d, t.w, t.wp, 0 - t.q), t.q);
+	}
 	default: {  // kPtModTwo, elemmodbytwo.comp:19-27
 		const uint64_t u = x & 1;
 		return x > t.bound ? 1 - u : u;
@@ -51,7 +58,7 @@ __device__ __forceinline__ uint64_t point_op(const PointTask &t, uint64_t x,
 // One workgroup row per task (blockIdx.y); unary ops never read b.
 __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks) {
 	const PointTask t = tasks[blockIdx.y];
-	const bool binary = t.op == kPtMul || t.op == kPtAdd;
+	const bool binary = t.op == kPtMul || t.op == kPtAdd || t.op == kPtRescale;
 	const uint64_t pairs = t.len >> 1;
 	const uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,9 +81,10 @@ void DeviceTasks::release() {
 	(void) hipFree(fwd);
 	(void) hipFree(inv);
 	(void) hipFree(pt);
+	(void) hipFree(post);
 	fwd = inv = nullptr;
-	pt = nullptr;
-	cap_fwd = cap_inv = cap_pt = 0;
+	pt = post = nullptr;
+	cap_fwd = cap_inv = cap_pt = cap_post = 0;
 }
 
 int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
@@ -111,13 +119,18 @@ int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 	return VKHEL_OK;
 }
 
+static const uint64_t kNoParams = 0;
+
 int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		vkhel_vector *res, vkhel_ntt_tables *const *tables, uint32_t nlimbs,
 		const uint64_t *params, Job *out) {
-	if (!a || !res || !tables || nlimbs == 0 || !params)
+	if (!a || !res || !tables || nlimbs == 0 ||
+			(!params && op != VKHEL_JOB_RESCALE))
 		return fail(VKHEL_ERR_INVALID, "rns job: null argument");
-	if (op < VKHEL_JOB_CENTER || op > VKHEL_JOB_INTT_CENTER)
+	if (op < VKHEL_JOB_CENTER || op > VKHEL_JOB_RESCALE)
 		return fail(VKHEL_ERR_INVALID, "rns job: unknown op %d", op);
+	if (op == VKHEL_JOB_RESCALE && !params)
+		params = &kNoParams;
 	if (a->ctx != ctx || res->ctx != ctx)
 		return fail(VKHEL_ERR_INVALID, "rns job: vector from another context");
 	for (uint32_t i = 0; i < nlimbs; i++)
@@ -125,12 +138,29 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 			return fail(VKHEL_ERR_INVALID,
 					"rns job: limb tables must all exist and share n");
 	const uint64_t words = (uint64_t) nlimbs * tables[0]->n;
-	if (a->length < words || res->length < words)
+	// rescaling drops the last limb from the result
+	const uint64_t res_words = op == VKHEL_JOB_RESCALE
+		? words - tables[0]->n : words;
+	if (a->length < words || res->length < res_words)
 		return fail(VKHEL_ERR_INVALID,
-				"rns job: vectors need %llu words (%u limbs x n)",
-				(unsigned long long) words, nlimbs);
+				"rns job: vectors need %llu and %llu words (%u limbs x n)",
+				(unsigned long long) words, (unsigned long long) res_words,
+				nlimbs);
+	if (op == VKHEL_JOB_RESCALE) {
+		if (nlimbs < 2)
+			return fail(VKHEL_ERR_INVALID, "rns job: rescale needs 2+ limbs");
+		const uint64_t ql = tables[nlimbs - 1]->q;
+		for (uint32_t i = 0; i + 1 < nlimbs; i++)
+			if (gcd_u64(ql % tables[i]->q, tables[i]->q) != 1)
+				return fail(VKHEL_ERR_INVALID,
+						"rns job: rescale needs the last modulus invertible mod "
+						"every other (limb %u)", i);
+		if (a == res)
+			return fail(VKHEL_ERR_INVALID,
+					"rns job: rescale reads the dropped limb; res must not be a");
+	}
 	const bool center = op == VKHEL_JOB_CENTER || op == VKHEL_JOB_INTT_CENTER;
-	const size_t np = center ? 1 :
+	const size_t np = op == VKHEL_JOB_RESCALE ? 0 : center ? 1 :
 		(op == VKHEL_JOB_GTADD ? 2 : 3) * (size_t) nlimbs;
 	if (center && params[0] < 2)
 		return fail(VKHEL_ERR_INVALID, "rns job: center needs p >= 2");
@@ -219,6 +249,20 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 			case VKHEL_JOB_NTT_INV:
 				ts.inv.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, true));
 				break;
+			case VKHEL_JOB_RESCALE: {
+				// limb l of (a - [a_last]) / q_last, after every transform
+				const size_t last = j.tables.size() - 1;
+				if (l == last)
+					break;
+				const uint64_t ql = j.tables[last]->q;
+				const uint64_t w = invmod_euclid(ql % q, q);
+				PointTask p{j.a->dev + off, j.a->dev + last * n, j.res->dev + off, n,
+					q, barrett_mu(q), ql / 2, ql % q, (uint32_t) (bitlen(q) - 2),
+					kPtRescale, w, shoup63_factor(w, q)};
+				ts.post.push_back(p);
+				ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
+				break;
+			}
 			case VKHEL_JOB_INTT_CENTER: {
 				// elemmod applied by the inverse kernel as it stores
 				NttTask inv = make_task(*t, j.a->dev + off, j.res->dev + off, true);
@@ -295,7 +339,8 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				break;
 			}
 			}
-			if (j.op >= VKHEL_JOB_MUL && j.op != VKHEL_JOB_INTT_CENTER)
+			if (j.op >= VKHEL_JOB_MUL && j.op != VKHEL_JOB_INTT_CENTER &&
+					j.op != VKHEL_JOB_RESCALE)
 				ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
 		}
 	}
@@ -325,7 +370,8 @@ static bool upload_array(const std::vector<T> &h, T **d, size_t *cap,
 int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s) {
 	if (!upload_array(ts.fwd, &dt.fwd, &dt.cap_fwd, s) ||
 			!upload_array(ts.inv, &dt.inv, &dt.cap_inv, s) ||
-			!upload_array(ts.pt, &dt.pt, &dt.cap_pt, s))
+			!upload_array(ts.pt, &dt.pt, &dt.cap_pt, s) ||
+			!upload_array(ts.post, &dt.post, &dt.cap_post, s))
 		return vkhel_ext_last_error();
 	return VKHEL_OK;
 }
@@ -348,19 +394,31 @@ static int launch_segments(const vkhel_ctx *ctx,
 	return VKHEL_OK;
 }
 
+// One pointwise phase: a row of workgroups per task, rows in launches of at
+// most 65535 (the grid's y limit).
+static int launch_pointwise(const std::vector<PointTask> &h, const PointTask *d,
+		uint64_t maxlen, hipStream_t s) {
+	const uint64_t blocks = std::max<uint64_t>(
+			std::min<uint64_t>((maxlen / 2 + 255) / 256, 64), 1);
+	for (size_t first = 0; first < h.size(); first += 65535) {
+		const size_t rows = std::min<size_t>(h.size() - first, 65535);
+		hipLaunchKernelGGL(k_pointwise_tasks, dim3((unsigned) blocks,
+				(unsigned) rows), dim3(256), 0, s, d + first);
+		if (!hip_ok(hipGetLastError(), "pointwise launch"))
+			return VKHEL_ERR_DEVICE;
+	}
+	return VKHEL_OK;
+}
+
 int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
 		const DeviceTasks &dt, hipStream_t s) {
 	int rc = launch_segments(ctx, ts.seg_fwd, ts.fwd, dt.fwd, false, s);
-	if (rc == VKHEL_OK && !ts.pt.empty()) {
-		const uint64_t blocks = std::min<uint64_t>((ts.pt_maxlen / 2 + 255) / 256,
-				64);
-		hipLaunchKernelGGL(k_pointwise_tasks, dim3((unsigned) std::max<uint64_t>(
-						blocks, 1), (unsigned) ts.pt.size()), dim3(256), 0, s, dt.pt);
-		rc = hip_ok(hipGetLastError(), "pointwise launch") ? VKHEL_OK
-			: VKHEL_ERR_DEVICE;
-	}
+	if (rc == VKHEL_OK)
+		rc = launch_pointwise(ts.pt, dt.pt, ts.pt_maxlen, s);
 	if (rc == VKHEL_OK)
 		rc = launch_segments(ctx, ts.seg_inv, ts.inv, dt.inv, true, s);
+	if (rc == VKHEL_OK)
+		rc = launch_pointwise(ts.post, dt.post, ts.post_maxlen, s);
 	return rc;
 }
 

@@ -1,7 +1,8 @@
 // Job model shared by the batch API (batch.hip) and the serving engine
 // (serve.hip): a job is one operation on an RNS polynomial (nlimbs limbs of
 // n words, limb i modulo tables[i]->q); a set of jobs is planned into three
-// device phases of tasks (forward transforms, pointwise, inverse).
+// device phases of tasks (forward transforms, pointwise, inverse), plus a
+// pointwise phase after the inverse for RNS rescaling.
 #pragma once
 
 #include <vector>
@@ -24,13 +25,18 @@ struct Job {
 //   kPtGtSub out = elemgtsub(a, bound, diff, q)  (also elemmod, p != 2)
 //   kPtGtAdd out = elemgtadd(a, bound, diff)
 //   kPtModTwo out = elemmod(a, 2, .) with bound = q_limb / 2
-enum { kPtMul = 0, kPtAdd, kPtGtSub, kPtGtAdd, kPtModTwo };
+//   kPtRescale out = (a - [b]) * w mod q, [b] the centered lift of the
+//              dropped limb b (b > bound: b - q_last) reduced mod q,
+//              diff = q_last mod q, w = q_last^-1 mod q (wp its packed
+//              Shoup companion)
+enum { kPtMul = 0, kPtAdd, kPtGtSub, kPtGtAdd, kPtModTwo, kPtRescale };
 struct PointTask {
 	const uint64_t *a, *b;
 	uint64_t *out;
 	uint64_t len, q, mu;
 	uint64_t bound, diff;
 	uint32_t shift, op;
+	uint64_t w, wp;
 };
 
 // A run of transform tasks with one log2 n and one modulus class (small_q:
@@ -43,15 +49,15 @@ struct Segment {
 struct TaskSet {
 	std::vector<NttTask> fwd, inv;
 	std::vector<Segment> seg_fwd, seg_inv;
-	std::vector<PointTask> pt;
-	uint64_t pt_maxlen = 0;
+	std::vector<PointTask> pt, post;  // before / after the inverse phase
+	uint64_t pt_maxlen = 0, post_maxlen = 0;
 };
 
 // Device copies of a TaskSet's arrays, grown on demand.
 struct DeviceTasks {
 	NttTask *fwd = nullptr, *inv = nullptr;
-	PointTask *pt = nullptr;
-	size_t cap_fwd = 0, cap_inv = 0, cap_pt = 0;
+	PointTask *pt = nullptr, *post = nullptr;
+	size_t cap_fwd = 0, cap_inv = 0, cap_pt = 0, cap_post = 0;
 	void release();
 };
 
@@ -59,7 +65,8 @@ struct DeviceTasks {
 int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		const vkhel_vector *b, vkhel_vector *res,
 		vkhel_ntt_tables *const *tables, uint32_t nlimbs, Job *out);
-// Checks an RNS elementwise job (VKHEL_JOB_CENTER / _GTADD / _GTSUB).
+// Checks an RNS job (VKHEL_JOB_CENTER / _GTADD / _GTSUB / _INTT_CENTER /
+// _RESCALE).
 int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		vkhel_vector *res, vkhel_ntt_tables *const *tables, uint32_t nlimbs,
 		const uint64_t *params, Job *out);

@@ -16,6 +16,7 @@ NTT_AUTO, NTT_RADIX2, NTT_FUSED = 0, 1, 2
 JOB_NTT_FWD, JOB_NTT_INV, JOB_POLYMUL, JOB_MUL, JOB_ADD = range(5)
 JOB_CENTER, JOB_GTADD, JOB_GTSUB = 5, 6, 7  # RNS elementwise (add_rns)
 JOB_INTT_CENTER = 8  # fused INTT + centered lift (add_rns, params [p])
+JOB_RESCALE = 9  # drop the last limb, rounding (add_rns, no params)
 
 
 def _L():
@@ -321,15 +322,16 @@ class Batch:
                                         b.handle if b is not None else None,
                                         res.handle, arr, len(tabs)))
 
-    def add_rns(self, op, a, res, tables, params):
+    def add_rns(self, op, a, res, tables, params=None):
         """vkhel_batch_add_rns: JOB_CENTER or JOB_INTT_CENTER (params [p]),
-        JOB_GTADD ([bound_i, diff_i] per limb) or JOB_GTSUB
-        ([bound_i, diff_i, mod_i])."""
+        JOB_GTADD ([bound_i, diff_i] per limb), JOB_GTSUB
+        ([bound_i, diff_i, mod_i]) or JOB_RESCALE (no params)."""
         tabs = list(tables)
         arr = (ctypes.c_void_p * len(tabs))(*[t.handle for t in tabs])
-        prm = np.ascontiguousarray(params, dtype=np.uint64)
+        prm = None if params is None else np.ascontiguousarray(params, dtype=np.uint64)
         _lib.check(_L().vkhel_batch_add_rns(self._h, op, a.handle, res.handle,
-                                            arr, len(tabs), _u64p(prm)))
+                                            arr, len(tabs),
+                                            None if prm is None else _u64p(prm)))
 
     def clear(self):
         _L().vkhel_batch_clear(self._h)
