@@ -651,14 +651,18 @@ def traffic_from_profile(logn, npoly, tile_bits):
     return prof.get("hbm_bytes_per_launch"), prof
 
 
-def fwd_kernel_names(logn, tile_bits, pass_mode, small_q):
-    """The dispatches of one forward transform at this size and tuning
-    (moduli below 2^56 use the small-modulus kernels)."""
-    nr = ", true" if small_q else ""
+def modulus_class(q):
+    """The kernels' modulus class: 2 below 2^56, 1 below 2^60, else 0."""
+    return 2 if q < (1 << 56) else 1 if q < (1 << 60) else 0
+
+
+def fwd_kernel_names(logn, tile_bits, pass_mode, mc):
+    """The dispatches of one forward transform at this size and tuning for
+    moduli of class mc."""
     if logn <= tile_bits or pass_mode == 1:
-        return [f"k_ntt_fused<{logn}, false, {tile_bits}{nr}>"]
-    return [f"k_ntt_pass<{logn}, 1, false, {tile_bits}{nr}>",
-            f"k_ntt_pass<{logn}, 2, false, {tile_bits}{nr}>"]
+        return [f"k_ntt_fused<{logn}, false, {tile_bits}, {mc}>"]
+    return [f"k_ntt_pass<{logn}, 1, false, {tile_bits}, {mc}>",
+            f"k_ntt_pass<{logn}, 2, false, {tile_bits}, {mc}>"]
 
 
 def main():
@@ -767,7 +771,7 @@ def main():
                      "traffic": traffic,
                      "kernel": "forward NTT of the batch: " + " + ".join(
                          fwd_kernel_names(args.logn, tb, ctx.pass_mode(args.logn),
-                                          max(qs) < (1 << 56))),
+                                          modulus_class(max(qs)))),
                      "kernel_ms": round(r["fwd_ms"], 4),
                      "inverse_kernel_ms": round(r["inv_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes,

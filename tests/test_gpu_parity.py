@@ -383,15 +383,16 @@ def test_batch_roundtrip_full_size(ctx):
 
 # ---------------- tile sizes, split vs whole-polynomial kernels ----------------
 
+@pytest.mark.parametrize("bits", [55, 59])
 @pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("tb", [11, 12, 13])
 @pytest.mark.parametrize("logn", [12, 13, 14, 16, 17, 18, 19, 20])
-def test_tile_sizes_per_call_and_batched(ctx, logn, tb, mode):
-    """Every compiled tile size in both pass modes: 1 = one workgroup per
+def test_tile_sizes_per_call_and_batched(ctx, logn, tb, mode, bits):
+    """Every compiled tile size in both pass modes (1 = one workgroup per
     polynomial running both passes, 2 = one grid per pass, one workgroup per
-    tile."""
+    tile) for moduli of the small (55-bit) and mid (59-bit) classes."""
     n = 1 << logn
-    q = _prime(logn, 55)
+    q = _prime(logn, bits)
     w = O.root_2n(q, n)
     t = V.NttTables(n, q, w)
     ctx.set_tile_bits(logn, tb)
@@ -408,7 +409,7 @@ def test_tile_sizes_per_call_and_batched(ctx, logn, tb, mode):
         assert np.array_equal(f.read(), x)
         # a batch through the task array (80 polynomials; 6 at 2^19 and up)
         npoly = 80 if logn <= 18 else 6
-        rng = np.random.default_rng(logn * 100 + tb)
+        rng = np.random.default_rng(logn * 100 + tb + bits)
         xs = [rng.integers(0, q, n, dtype=np.uint64) for _ in range(npoly)]
         vs = [vec(ctx, z) for z in xs]
         fb, ib = V.Batch(ctx), V.Batch(ctx)

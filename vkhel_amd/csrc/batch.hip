@@ -208,16 +208,16 @@ uint64_t job_transform_coeffs(const Job &j) {
 // Sorts tasks by log2 n and modulus class and records the runs.
 static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
 	const auto key = [](const NttTask &k) {
-		return 2 * k.c.logn + (k.c.q < kNoReduceQ ? 1 : 0);
+		return 3 * k.c.logn + (uint32_t) modulus_class(k.c.q);
 	};
 	std::stable_sort(tasks.begin(), tasks.end(),
 			[&](const NttTask &a, const NttTask &b) { return key(a) < key(b); });
 	std::vector<Segment> segs;
 	for (uint32_t i = 0; i < tasks.size(); i++) {
-		const bool small_q = tasks[i].c.q < kNoReduceQ;
+		const int mc = modulus_class(tasks[i].c.q);
 		if (segs.empty() || segs.back().logn != tasks[i].c.logn ||
-				segs.back().small_q != small_q)
-			segs.push_back(Segment{i, 0, tasks[i].c.logn, small_q});
+				segs.back().mc != mc)
+			segs.push_back(Segment{i, 0, tasks[i].c.logn, mc});
 		segs.back().count++;
 	}
 	return segs;
@@ -383,7 +383,7 @@ static int launch_segments(const vkhel_ctx *ctx,
 		int rc = VKHEL_OK;
 		if (ntt_fused_supported(g.logn)) {
 			rc = launch_fused_tasks(ctx, dev + g.off, g.count, g.logn, inverse,
-					g.small_q, s);
+					g.mc, s);
 		} else {
 			for (uint32_t i = 0; i < g.count && rc == VKHEL_OK; i++)
 				rc = launch_task_ntt(host[g.off + i], inverse, s);

@@ -142,6 +142,14 @@ __host__ __device__ __forceinline__ uint32_t tw_pos(uint32_t m, uint32_t i) {
 // after the last stage of n = 2^20; the inverse defers its sum reductions
 // (bounds below 64q).  The 63-bit Shoup product accepts all of these.
 constexpr uint64_t kNoReduceQ = 1ull << 56;
+// Below this bound (and from kNoReduceQ up) the mid class: forward values
+// kept below 8q with one reduction per butterfly, inverse below 4q, both
+// with products below 4q (8q < 2^63).
+constexpr uint64_t kMidQ = 1ull << 60;
+enum { kClassGeneral = 0, kClassMid = 1, kClassSmall = 2 };
+inline int modulus_class(uint64_t q) {
+	return q < kNoReduceQ ? kClassSmall : q < kMidQ ? kClassMid : kClassGeneral;
+}
 
 struct NttTask {
 	const uint64_t *in;
@@ -161,11 +169,10 @@ int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
 NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
 		bool inverse);
 // Device-resident tasks that share log2 n (which selects the compiled
-// kernel), direction and modulus class (small_q: every q below kNoReduceQ);
+// kernel), direction and modulus class (mc, modulus_class() of every q);
 // the context supplies the tuned tile size.
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
-		uint32_t ntasks, uint32_t logn, bool inverse, bool small_q,
-		hipStream_t st);
+		uint32_t ntasks, uint32_t logn, bool inverse, int mc, hipStream_t st);
 // Tile size (log2 words) used for log2 n on this context.
 int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn);
 bool ntt_fused_supported(uint32_t logn);

@@ -39,11 +39,11 @@ struct PointTask {
 	uint64_t w, wp;
 };
 
-// A run of transform tasks with one log2 n and one modulus class (small_q:
-// every q below kNoReduceQ).
+// A run of transform tasks with one log2 n and one modulus class
+// (modulus_class() of every q).
 struct Segment {
 	uint32_t off, count, logn;
-	bool small_q;
+	int mc;
 };
 
 struct TaskSet {

@@ -187,20 +187,16 @@ int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn) {
 // passes as separate grids of one workgroup per tile.
 static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
 		uint32_t ntasks, uint32_t logn, int tile_bits, bool split, bool inverse,
-		bool small_q, hipStream_t st) {
+		int mc, hipStream_t st) {
 	const FusedEntry &e = fused_entry(logn, tile_bits);
-	const bool nr = small_q;
 	if (split && e.tiles > 1) {
 		const dim3 grid(e.tiles, ntasks);
-		hipLaunchKernelGGL(inverse ? (nr ? e.inv_b_nr : e.inv_b)
-				: nr ? e.fwd_a_nr : e.fwd_a, grid,
+		hipLaunchKernelGGL(inverse ? e.inv_b[mc] : e.fwd_a[mc], grid,
 				dim3(e.threads), 0, st, uniform, dev_tasks, 0u);
-		hipLaunchKernelGGL(inverse ? (nr ? e.inv_a_nr : e.inv_a)
-				: nr ? e.fwd_b_nr : e.fwd_b, grid,
+		hipLaunchKernelGGL(inverse ? e.inv_a[mc] : e.fwd_b[mc], grid,
 				dim3(e.threads), 0, st, uniform, dev_tasks, 1u);
 	} else {
-		hipLaunchKernelGGL(inverse ? (nr ? e.inv_nr : e.inv)
-				: nr ? e.fwd_nr : e.fwd,
+		hipLaunchKernelGGL(inverse ? e.inv[mc] : e.fwd[mc],
 				dim3(ntasks), dim3(e.threads), 0, st, uniform, dev_tasks);
 	}
 	return hip_ok(hipGetLastError(), "ntt fused launch") ? VKHEL_OK
@@ -217,15 +213,14 @@ static bool use_split(const vkhel_ctx *ctx, uint32_t logn) {
 }
 
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
-		uint32_t ntasks, uint32_t logn, bool inverse, bool small_q,
-		hipStream_t st) {
+		uint32_t ntasks, uint32_t logn, bool inverse, int mc, hipStream_t st) {
 	if (ntasks == 0)
 		return VKHEL_OK;
 	if (!fused_supported(logn))
 		return fail(VKHEL_ERR_INVALID, "fused NTT: unsupported log2 n = %u", logn);
 	const NttTask none{};
 	return launch_fused_impl(none, dev_tasks, ntasks, logn,
-			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, small_q, st);
+			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, mc, st);
 }
 
 int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
@@ -237,7 +232,7 @@ int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
 		return launch_radix2(t, in, out, batch, inverse, s);
 	return launch_fused_impl(make_task(t, in, out, inverse), nullptr, batch, logn,
 			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse,
-			t.c.q < kNoReduceQ, s);
+			modulus_class(t.c.q), s);
 }
 
 static void transform(const vkhel_vector *op, vkhel_vector *res,

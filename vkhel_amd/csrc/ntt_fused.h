@@ -14,17 +14,14 @@ typedef void (*FusedKernel)(NttTask, const NttTask *);
 
 typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
 
-// One compiled (log2 n, tile size) variant: whole-transform kernels for both
-// directions (one workgroup per polynomial), the two passes as separate
-// kernels (one workgroup per tile: the default for two-pass sizes), and
-// launch shape.
+// One compiled (log2 n, tile size) variant, per modulus class (index
+// kClassGeneral / kClassMid / kClassSmall, modulus_class()): whole-transform
+// kernels for both directions (one workgroup per polynomial), the two
+// passes as separate kernels (one workgroup per tile: the default for
+// two-pass sizes), and launch shape.
 struct FusedEntry {
-	FusedKernel fwd, inv;
-	PassKernel fwd_a, fwd_b, inv_b, inv_a;
-	// kernels for q below kNoReduceQ: forward without the butterflies' input
-	// reduction, inverse with the sums reduced once per round
-	FusedKernel fwd_nr, inv_nr;
-	PassKernel fwd_a_nr, fwd_b_nr, inv_b_nr, inv_a_nr;
+	FusedKernel fwd[3], inv[3];
+	PassKernel fwd_a[3], fwd_b[3], inv_b[3], inv_a[3];
 	int threads;
 	int tiles;  // tiles per pass (1: one-tile size, passes unused)
 };
@@ -42,12 +39,12 @@ constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
 // and its packed 63-bit Shoup companion (shoup63_factor).
 struct LazyK {
 	uint64_t two_q, n_two_q, nq;  // 2q, 2^64 - 2q, 2^64 - q
-	uint64_t four_q;
+	uint64_t four_q, n_four_q;
 };
 
 __device__ __forceinline__ LazyK lazy_consts(const NttConsts &c) {
 	return LazyK{c.two_q, opaque_uniform(0 - c.two_q), opaque_uniform(0 - c.q),
-		c.two_q << 1};
+		c.two_q << 1, opaque_uniform(0 - (c.two_q << 1))};
 }
 
 // 2^64 - 2^h q, h = 3..5, for the inverse's end-of-round reductions.
@@ -93,23 +90,45 @@ __device__ __forceinline__ void bf_inv(uint64_t &x, uint64_t &y, ulonglong2 w,
 
 // First inverse stage: the inverse's inputs are below q (above q the
 // reference's butterfly, nttrevbutterfly.comp:44-48, is not meaningful), so
-// the sum is already below 2q and needs no reduction.
+// the sum is already below 2q and needs no reduction.  LAZY: the mid
+// class's product, below 4q.
+template <bool LAZY>
 __device__ __forceinline__ void bf_inv_first(uint64_t &x, uint64_t &y,
 		ulonglong2 w, const LazyK &k) {
 	const uint64_t s = x + y;
 	const uint64_t d = x + k.two_q - y;
 	x = s;
-	y = shoup63(d, w.x, w.y, k.nq);
+	y = LAZY ? shoup63_lazy(d, w.x, w.y, k.nq) : shoup63(d, w.x, w.y, k.nq);
+}
+
+// Mid-class forward butterfly (q < kMidQ): x, y in [0, 8q) -> [0, 8q),
+// with the input reduced below 4q and a shoup63_lazy product (below 4q):
+// 8q < 2^63 keeps the product's input in range.
+__device__ __forceinline__ void bf_fwd_mid(uint64_t &x, uint64_t &y,
+		ulonglong2 w, const LazyK &k) {
+	const uint64_t xr = csub_n(x, k.four_q, k.n_four_q);
+	const uint64_t t = shoup63_lazy(y, w.x, w.y, k.nq);
+	x = xr + t;
+	y = xr + k.four_q - t;
+}
+
+// Mid-class inverse butterfly: x, y in [0, 4q) -> [0, 4q).
+__device__ __forceinline__ void bf_inv_mid(uint64_t &x, uint64_t &y,
+		ulonglong2 w, const LazyK &k) {
+	const uint64_t s = csub_n(x + y, k.four_q, k.n_four_q);
+	const uint64_t d = x + k.four_q - y;
+	x = s;
+	y = shoup63_lazy(d, w.x, w.y, k.nq);
 }
 
 // First forward stage when both inputs are already below q: the lazy
-// butterfly without its input reduction (with NR, the one of bf_fwd_nr).
-// The reference's first stage gives the canonical forms of these same
-// residues.
-template <bool NR>
+// butterfly without its input reduction (for the small and mid classes the
+// product of shoup63_lazy, below 4q).  The reference's first stage gives
+// the canonical forms of these same residues.
+template <int MC>
 __device__ __forceinline__ void bf_fwd_first_canon(uint64_t &x, uint64_t &y,
 		ulonglong2 w, const LazyK &k) {
-	if constexpr (NR) {
+	if constexpr (MC != kClassGeneral) {
 		bf_fwd_nr(x, y, w, k);
 	} else {
 		const uint64_t t = shoup63(y, w.x, w.y, k.nq);
@@ -457,8 +476,9 @@ constexpr bool round_wave_local() {
 }
 
 // TRAIL: end with a workgroup barrier, for callers that reuse the LDS tile.
-// NR: forward butterflies without input reduction (q < kNoReduceQ).
-template <class G, bool INV, int R, int LOGN, bool TRAIL, bool NR>
+// MC: modulus class (modulus_class(q)); the small class's forward
+// butterflies run without input reduction.
+template <class G, bool INV, int R, int LOGN, bool TRAIL, int MC>
 __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
 		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
@@ -526,6 +546,8 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		// inverse with deferred sum reductions (q < kNoReduceQ): stages
 		// present in this round, and the bound its inputs enter with (below
 		// q at the transform's first round, below 8q after any other)
+		constexpr bool NR = MC == kClassSmall;
+		constexpr bool MID = MC == kClassMid;
 		constexpr bool LAZY_INV = INV && NR;
 		constexpr int ACTIVE = (F >= RD::BLO && F < RD::BHI ? 1 : 0) |
 			(F + 1 >= RD::BLO && F + 1 < RD::BHI ? 2 : 0) |
@@ -552,7 +574,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 				if (__all(canon)) {
 #pragma unroll
 					for (int e = 0; e < 4; e++)
-						bf_fwd_first_canon<NR>(v[e], v[e + 4], w[0], lk);
+						bf_fwd_first_canon<MC>(v[e], v[e + 4], w[0], lk);
 				} else {
 #pragma unroll
 					for (int e = 0; e < 4; e++)
@@ -564,6 +586,8 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 					if constexpr (LAZY_INV)
 						bf_inv_last_nr(v[e], v[e + 4], c.q << log2_exact(
 								inv_bound(BIN, ACTIVE, d - 1, e + 4)), c, lk);
+					else if constexpr (MID)
+						bf_inv_last_nr(v[e], v[e + 4], lk.four_q, c, lk);
 					else
 						bf_inv_last(v[e], v[e + 4], c, lk);
 				}
@@ -578,11 +602,16 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 									c.q << log2_exact(inv_bound(BIN, ACTIVE, d - 1,
 											e | (1 << d))), lk);
 						else if (INV && b == 0)
-							bf_inv_first(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+							bf_inv_first<MID>(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
+									lk);
+						else if (INV && MID)
+							bf_inv_mid(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 						else if (INV)
 							bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 						else if (NR)
 							bf_fwd_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+						else if (MID)
+							bf_fwd_mid(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 						else
 							bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
 					}
@@ -614,9 +643,10 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 			}
 		}
 		if (!INV && LAST_ROUND && final) {
-			if (NR) {
-				// from below 81q: a quotient estimate off by at most one
-				// (small_reduce_consts), then one subtraction
+			if (NR || MID) {
+				// from below 81q (small class) or 8q (mid): a quotient
+				// estimate off by at most one (small_reduce_consts), then one
+				// subtraction
 #pragma unroll
 				for (int e = 0; e < 8; e++) {
 					const uint32_t qe = __umulhi((uint32_t) (v[e] >> c.red_shift),
@@ -669,18 +699,18 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	}
 }
 
-template <class G, bool INV, int LOGN, bool TRAIL, bool NR, int... R>
+template <class G, bool INV, int LOGN, bool TRAIL, int MC, int... R>
 __device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...>,
 		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
 		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
 		const NttConsts &c, bool final) {
 	uint64_t v[8];
 	ulonglong2 w[7];
-	(run_round<G, INV, R, LOGN, TRAIL, NR>(v, w, t, src, src2, dst, lds, jbase,
+	(run_round<G, INV, R, LOGN, TRAIL, MC>(v, w, t, src, src2, dst, lds, jbase,
 			gbase, tw, c, final), ...);
 }
 
-template <int LOGN, int PASS, int TB, bool INV, bool NR>
+template <int LOGN, int PASS, int TB, bool INV, int MC>
 __device__ __forceinline__ void run_pass(const uint64_t *src,
 		const uint64_t *src2, uint64_t *dst, uint64_t *lds, const ulonglong2 *tw,
 		const NttConsts &c, bool final) {
@@ -691,7 +721,7 @@ __device__ __forceinline__ void run_pass(const uint64_t *src,
 		// i (2^TB contiguous words), which is also their twiddle offset.
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
-		run_rounds<G, INV, LOGN, true, NR>(
+		run_rounds<G, INV, LOGN, true, MC>(
 				std::make_integer_sequence<int, G::ROUNDS>(), t, src, src2, dst, lds,
 				jbase, gbase, tw, c, final);
 	}
@@ -706,7 +736,7 @@ struct FusedShape {
 
 // One workgroup per task.  With `tasks` null, workgroup i transforms the
 // i-th polynomial after `uniform` (same tables, contiguous polynomials).
-template <int LOGN, bool INV, int TB, bool NR = false>
+template <int LOGN, bool INV, int TB, int MC>
 __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 		NttTask uniform, const NttTask *tasks) {
 	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
@@ -722,15 +752,15 @@ __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 			k.in2 += off;
 	}
 	if constexpr (LOGN <= TB) {
-		run_pass<LOGN, 0, TB, INV, NR>(k.in, k.in2, k.out, lds, k.tw, k.c, true);
+		run_pass<LOGN, 0, TB, INV, MC>(k.in, k.in2, k.out, lds, k.tw, k.c, true);
 	} else if constexpr (!INV) {
-		run_pass<LOGN, 1, TB, INV, NR>(k.in, nullptr, k.out, lds, k.tw, k.c,
+		run_pass<LOGN, 1, TB, INV, MC>(k.in, nullptr, k.out, lds, k.tw, k.c,
 				false);
-		run_pass<LOGN, 2, TB, INV, NR>(k.out, nullptr, k.out, lds, k.tw, k.c,
+		run_pass<LOGN, 2, TB, INV, MC>(k.out, nullptr, k.out, lds, k.tw, k.c,
 				true);
 	} else {
-		run_pass<LOGN, 2, TB, INV, NR>(k.in, k.in2, k.out, lds, k.tw, k.c, false);
-		run_pass<LOGN, 1, TB, INV, NR>(k.out, nullptr, k.out, lds, k.tw, k.c,
+		run_pass<LOGN, 2, TB, INV, MC>(k.in, k.in2, k.out, lds, k.tw, k.c, false);
+		run_pass<LOGN, 1, TB, INV, MC>(k.out, nullptr, k.out, lds, k.tw, k.c,
 				true);
 	}
 }
@@ -745,7 +775,7 @@ constexpr int pass_waves(int logn, int pass, int tb) {
 	return logn == 20 && pass == 1 && tb == 13 ? 5 : 8;
 }
 
-template <int LOGN, int PASS, bool INV, int TB, bool NR = false>
+template <int LOGN, int PASS, bool INV, int TB, int MC>
 __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
 __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass(
 		NttTask uniform, const NttTask *tasks, uint32_t second) {
@@ -766,7 +796,7 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 		const uint32_t i = blockIdx.x;
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
-		run_rounds<G, INV, LOGN, false, NR>(
+		run_rounds<G, INV, LOGN, false, MC>(
 				std::make_integer_sequence<int, G::ROUNDS>(), threadIdx.x,
 				second ? k.out : k.in, second ? nullptr : k.in2, k.out, lds, jbase,
 				gbase, k.tw, k.c, second != 0);
@@ -777,18 +807,25 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 template <int TB, int... L>
 constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 		std::integer_sequence<int, L...>) {
-	return {FusedEntry{k_ntt_fused<L + kMinFusedLogn, false, TB>,
-		k_ntt_fused<L + kMinFusedLogn, true, TB>,
-		k_ntt_pass<L + kMinFusedLogn, 1, false, TB>,
-		k_ntt_pass<L + kMinFusedLogn, 2, false, TB>,
-		k_ntt_pass<L + kMinFusedLogn, 2, true, TB>,
-		k_ntt_pass<L + kMinFusedLogn, 1, true, TB>,
-		k_ntt_fused<L + kMinFusedLogn, false, TB, true>,
-		k_ntt_fused<L + kMinFusedLogn, true, TB, true>,
-		k_ntt_pass<L + kMinFusedLogn, 1, false, TB, true>,
-		k_ntt_pass<L + kMinFusedLogn, 2, false, TB, true>,
-		k_ntt_pass<L + kMinFusedLogn, 2, true, TB, true>,
-		k_ntt_pass<L + kMinFusedLogn, 1, true, TB, true>,
+	return {FusedEntry{
+		{k_ntt_fused<L + kMinFusedLogn, false, TB, 0>,
+			k_ntt_fused<L + kMinFusedLogn, false, TB, 1>,
+			k_ntt_fused<L + kMinFusedLogn, false, TB, 2>},
+		{k_ntt_fused<L + kMinFusedLogn, true, TB, 0>,
+			k_ntt_fused<L + kMinFusedLogn, true, TB, 1>,
+			k_ntt_fused<L + kMinFusedLogn, true, TB, 2>},
+		{k_ntt_pass<L + kMinFusedLogn, 1, false, TB, 0>,
+			k_ntt_pass<L + kMinFusedLogn, 1, false, TB, 1>,
+			k_ntt_pass<L + kMinFusedLogn, 1, false, TB, 2>},
+		{k_ntt_pass<L + kMinFusedLogn, 2, false, TB, 0>,
+			k_ntt_pass<L + kMinFusedLogn, 2, false, TB, 1>,
+			k_ntt_pass<L + kMinFusedLogn, 2, false, TB, 2>},
+		{k_ntt_pass<L + kMinFusedLogn, 2, true, TB, 0>,
+			k_ntt_pass<L + kMinFusedLogn, 2, true, TB, 1>,
+			k_ntt_pass<L + kMinFusedLogn, 2, true, TB, 2>},
+		{k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 0>,
+			k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 1>,
+			k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 2>},
 		FusedShape<L + kMinFusedLogn, TB>::THREADS,
 		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0)}...};
 }
