@@ -358,6 +358,47 @@ def test_polymul_rns_vs_oracle(ctx, logn, mode):
             t.close()
 
 
+@pytest.mark.parametrize("tb", [11, 12, 13])
+def test_fused_polymul_tiles(ctx, tb):
+    """Polymul jobs run a's forward, the product and the inverse in one
+    kernel (after a's column pass for two-pass sizes): every tile size, one-
+    and two-pass sizes, the three modulus classes (50/55-, 59- and 61-bit
+    primes) and a 24-bit prime, in one batch, against the oracle's product."""
+    lognes = (3, 5, 9, 11, 12, 13, 14, 15, 17, 19)
+    jobs = []
+    try:
+        for logn in lognes:
+            ctx.set_tile_bits(logn, tb)
+        batch = V.Batch(ctx)
+        for logn in lognes:
+            n = 1 << logn
+            bits = (50, 55, 59, 61) + ((24,) if logn <= 14 else ())
+            qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in bits]
+            tabs = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+            xa = np.concatenate([O.fill_uniform(n, q, 30 + i) for i, q in enumerate(qs)])
+            xb = np.concatenate([O.fill_uniform(n, q, 40 + i) for i, q in enumerate(qs)])
+            a, b, r = vec(ctx, xa), vec(ctx, xb), ctx.vector(len(qs) * n)
+            batch.add(V.JOB_POLYMUL, a, b, r, tabs)
+            jobs.append((logn, qs, tabs, xa, xb, a, b, r))
+        batch.run()
+        for logn, qs, tabs, xa, xb, a, b, r in jobs:
+            n = 1 << logn
+            got = r.read()
+            for i, q in enumerate(qs):
+                want = O.polymul_ntt(xa[i * n:(i + 1) * n], xb[i * n:(i + 1) * n],
+                                     q, tabs[i].w)
+                assert np.array_equal(got[i * n:(i + 1) * n], want), (tb, logn, q)
+            # the operands are left as they were
+            assert np.array_equal(a.read(), xa) and np.array_equal(b.read(), xb)
+        batch.close()
+    finally:
+        for logn in lognes:
+            ctx.set_tile_bits(logn, 0)
+        for job in jobs:
+            for t in job[2]:
+                t.close()
+
+
 def test_batch_roundtrip_full_size(ctx):
     # 2^16 x 32 polynomials, 4 primes: NTT then INTT is the identity.
     n, per, L = 1 << 16, 8, 4

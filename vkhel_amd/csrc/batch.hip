@@ -4,7 +4,10 @@
 // log2(n) fenced submits.  Here a job list turns into three device phases,
 // each a handful of launches over every polynomial limb of every job:
 //
-//   1. forward transforms   (NTT_FWD jobs, both operands of POLYMUL jobs)
+//   1. forward transforms   (NTT_FWD jobs, both operands of POLYMUL jobs,
+//                            only b where the polymul kernel is fused)
+//      fused polymuls       (a's forward, the product and the inverse in
+//                            one kernel: k_ntt_pm)
 //   2. pointwise products   (MUL, ADD jobs; POLYMUL spectra only for sizes
 //                            the fused kernels do not cover)
 //   3. inverse transforms   (NTT_INV jobs, POLYMUL results; the fused
@@ -80,11 +83,13 @@ __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks)
 void DeviceTasks::release() {
 	(void) hipFree(fwd);
 	(void) hipFree(inv);
+	(void) hipFree(pm_col);
+	(void) hipFree(pm);
 	(void) hipFree(pt);
 	(void) hipFree(post);
-	fwd = inv = nullptr;
+	fwd = inv = pm_col = pm = nullptr;
 	pt = post = nullptr;
-	cap_fwd = cap_inv = cap_pt = cap_post = 0;
+	cap_fwd = cap_inv = cap_pm_col = cap_pm = cap_pt = cap_post = 0;
 }
 
 int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
@@ -217,7 +222,7 @@ static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
 		const int mc = modulus_class(tasks[i].c.q);
 		if (segs.empty() || segs.back().logn != tasks[i].c.logn ||
 				segs.back().mc != mc)
-			segs.push_back(Segment{i, 0, tasks[i].c.logn, mc});
+			segs.push_back(Segment{i, 0, tasks[i].c.logn, mc, 0});
 		segs.back().count++;
 	}
 	return segs;
@@ -278,6 +283,23 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 			case VKHEL_JOB_POLYMUL: {
 				uint64_t *tmp = sp;
 				sp += n;
+				if (polymul_fused_tile(ctx, t->c.logn)) {
+					// b's spectrum, then a's forward, the product and the
+					// inverse in one kernel (after a's forward column pass
+					// for two-pass sizes, which runs in place in res)
+					ts.fwd.push_back(make_task(*t, j.b->dev + off, tmp, false));
+					const bool cols = t->c.logn > (uint32_t) polymul_fused_tile(ctx,
+							t->c.logn);
+					if (cols)
+						ts.pm_col.push_back(make_task(*t, j.a->dev + off,
+								j.res->dev + off, false));
+					NttTask pm = make_task(*t, cols ? j.res->dev + off
+							: j.a->dev + off, j.res->dev + off, true);
+					pm.in2 = tmp;
+					pm.tw2 = t->fwd;
+					ts.pm.push_back(pm);
+					break;
+				}
 				ts.fwd.push_back(make_task(*t, j.a->dev + off, j.res->dev + off, false));
 				ts.fwd.push_back(make_task(*t, j.b->dev + off, tmp, false));
 				NttTask inv = make_task(*t, j.res->dev + off, j.res->dev + off, true);
@@ -346,6 +368,12 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 	}
 	ts.seg_fwd = segment(ts.fwd);
 	ts.seg_inv = segment(ts.inv);
+	ts.seg_pm_col = segment(ts.pm_col);
+	ts.seg_pm = segment(ts.pm);
+	for (auto *segs : {&ts.seg_pm_col, &ts.seg_pm}) {
+		for (Segment &g : *segs)
+			g.tb = polymul_fused_tile(ctx, g.logn);
+	}
 	return VKHEL_OK;
 }
 
@@ -370,6 +398,8 @@ static bool upload_array(const std::vector<T> &h, T **d, size_t *cap,
 int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s) {
 	if (!upload_array(ts.fwd, &dt.fwd, &dt.cap_fwd, s) ||
 			!upload_array(ts.inv, &dt.inv, &dt.cap_inv, s) ||
+			!upload_array(ts.pm_col, &dt.pm_col, &dt.cap_pm_col, s) ||
+			!upload_array(ts.pm, &dt.pm, &dt.cap_pm, s) ||
 			!upload_array(ts.pt, &dt.pt, &dt.cap_pt, s) ||
 			!upload_array(ts.post, &dt.post, &dt.cap_post, s))
 		return vkhel_ext_last_error();
@@ -410,9 +440,24 @@ static int launch_pointwise(const std::vector<PointTask> &h, const PointTask *d,
 	return VKHEL_OK;
 }
 
+static int launch_polymul_segments(const std::vector<Segment> &segs,
+		const NttTask *dev, bool column_pass, hipStream_t s) {
+	for (const Segment &g : segs) {
+		const int rc = launch_polymul_tasks(dev + g.off, g.count, g.logn, g.tb,
+				g.mc, column_pass, s);
+		if (rc != VKHEL_OK)
+			return rc;
+	}
+	return VKHEL_OK;
+}
+
 int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
 		const DeviceTasks &dt, hipStream_t s) {
 	int rc = launch_segments(ctx, ts.seg_fwd, ts.fwd, dt.fwd, false, s);
+	if (rc == VKHEL_OK)
+		rc = launch_polymul_segments(ts.seg_pm_col, dt.pm_col, true, s);
+	if (rc == VKHEL_OK)
+		rc = launch_polymul_segments(ts.seg_pm, dt.pm, false, s);
 	if (rc == VKHEL_OK)
 		rc = launch_pointwise(ts.pt, dt.pt, ts.pt_maxlen, s);
 	if (rc == VKHEL_OK)

@@ -160,6 +160,8 @@ struct NttTask {
 	// input is the pointwise product in[i] * in2[i] mod q of two canonical
 	// spectra, formed as they are loaded (a polynomial product's last step).
 	const uint64_t *in2;
+	// Forward twiddles of a fused polynomial-product task (tw: inverse).
+	const ulonglong2 *tw2;
 };
 
 // Launchers (ntt.hip).  `batch` polynomials of n words, poly p at
@@ -173,6 +175,17 @@ NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
 // the context supplies the tuned tile size.
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
 		uint32_t ntasks, uint32_t logn, bool inverse, int mc, hipStream_t st);
+// Fused polynomial products (k_ntt_pm): the tile size (log2 words) they
+// run on for log2 n on this context, 0 when that size and tile do not allow
+// them.
+int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn);
+// Launches one stage of fused polynomial products on 2^tile_bits tiles,
+// tasks {in, out, tw = inverse twiddles, tw2 = forward twiddles, in2 = b's
+// spectrum}.  column_pass: the forward column pass of a (in -> out) of a
+// two-pass size; otherwise the fused forward rows, product and inverse rows
+// (reading in) and, for two-pass sizes, the inverse column pass (out).
+int launch_polymul_tasks(const NttTask *dev_tasks, uint32_t ntasks,
+		uint32_t logn, int tile_bits, int mc, bool column_pass, hipStream_t st);
 // Tile size (log2 words) used for log2 n on this context.
 int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn);
 bool ntt_fused_supported(uint32_t logn);

@@ -40,24 +40,31 @@ struct PointTask {
 };
 
 // A run of transform tasks with one log2 n and one modulus class
-// (modulus_class() of every q).
+// (modulus_class() of every q); tb: the tile size fused polynomial products
+// were planned for (0 for plain transforms, which take the context's).
 struct Segment {
 	uint32_t off, count, logn;
 	int mc;
+	int tb;
 };
 
 struct TaskSet {
 	std::vector<NttTask> fwd, inv;
 	std::vector<Segment> seg_fwd, seg_inv;
+	// fused polynomial products: the forward column passes of a (two-pass
+	// sizes), then the fused middle (launch_polymul_tasks)
+	std::vector<NttTask> pm_col, pm;
+	std::vector<Segment> seg_pm_col, seg_pm;
 	std::vector<PointTask> pt, post;  // before / after the inverse phase
 	uint64_t pt_maxlen = 0, post_maxlen = 0;
 };
 
 // Device copies of a TaskSet's arrays, grown on demand.
 struct DeviceTasks {
-	NttTask *fwd = nullptr, *inv = nullptr;
+	NttTask *fwd = nullptr, *inv = nullptr, *pm_col = nullptr, *pm = nullptr;
 	PointTask *pt = nullptr, *post = nullptr;
-	size_t cap_fwd = 0, cap_inv = 0, cap_pt = 0, cap_post = 0;
+	size_t cap_fwd = 0, cap_inv = 0, cap_pm_col = 0, cap_pm = 0;
+	size_t cap_pt = 0, cap_post = 0;
 	void release();
 };
 

@@ -156,6 +156,7 @@ NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
 	k.tw = inverse ? t.inv : t.fwd;
 	k.c = t.c;
 	k.in2 = nullptr;
+	k.tw2 = nullptr;
 	return k;
 }
 
@@ -221,6 +222,38 @@ int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
 	const NttTask none{};
 	return launch_fused_impl(none, dev_tasks, ntasks, logn,
 			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, mc, st);
+}
+
+int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn) {
+	if (!fused_supported(logn))
+		return 0;
+	const int tb = tile_bits_for(ctx, logn);
+	return fused_entry(logn, tb).pm_ok ? tb : 0;
+}
+
+int launch_polymul_tasks(const NttTask *dev_tasks, uint32_t ntasks,
+		uint32_t logn, int tile_bits, int mc, bool column_pass, hipStream_t st) {
+	if (ntasks == 0)
+		return VKHEL_OK;
+	const FusedEntry &e = fused_entry(logn, tile_bits);
+	if (!fused_supported(logn) || !e.pm_ok || (column_pass && e.tiles == 1))
+		return fail(VKHEL_ERR_INVALID,
+				"fused polymul: unsupported log2 n = %u on 2^%d tiles", logn,
+				tile_bits);
+	const NttTask none{};
+	const dim3 grid = e.tiles > 1 ? dim3(e.tiles, ntasks) : dim3(ntasks);
+	if (column_pass) {
+		hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, none,
+				dev_tasks, 0u);
+	} else {
+		hipLaunchKernelGGL(e.pm[mc], grid, dim3(e.threads), 0, st, none,
+				dev_tasks, 0u);
+		if (e.tiles > 1)
+			hipLaunchKernelGGL(e.inv_a[mc], grid, dim3(e.threads), 0, st, none,
+					dev_tasks, 1u);
+	}
+	return hip_ok(hipGetLastError(), "fused polymul launch") ? VKHEL_OK
+		: VKHEL_ERR_DEVICE;
 }
 
 int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,

@@ -22,6 +22,10 @@ typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
 struct FusedEntry {
 	FusedKernel fwd[3], inv[3];
 	PassKernel fwd_a[3], fwd_b[3], inv_b[3], inv_a[3];
+	// the middle of a polynomial product (k_ntt_pm; pm_ok: this size and
+	// tile allow it)
+	PassKernel pm[3];
+	bool pm_ok;
 	int threads;
 	int tiles;  // tiles per pass (1: one-tile size, passes unused)
 };
@@ -478,7 +482,12 @@ constexpr bool round_wave_local() {
 // TRAIL: end with a workgroup barrier, for callers that reuse the LDS tile.
 // MC: modulus class (modulus_class(q)); the small class's forward
 // butterflies run without input reduction.
-template <class G, bool INV, int R, int LOGN, bool TRAIL, int MC>
+// IO flags of a sequence of rounds: the first round's operands are
+// already in v in its own layout (no loads), and the last round leaves its
+// results in v (no stores).
+enum { kIoPreloaded = 1, kIoKeep = 2 };
+
+template <class G, bool INV, int R, int LOGN, bool TRAIL, int MC, int IO = 0>
 __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
 		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
@@ -488,6 +497,8 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	constexpr bool FIRST_ROUND = R == 0;
 	constexpr bool LAST_ROUND = R == G::ROUNDS - 1;
 	constexpr bool LOCAL = round_wave_local<G, INV, R>();
+	constexpr bool PRE = FIRST_ROUND && (IO & kIoPreloaded);
+	constexpr bool KEEP = LAST_ROUND && (IO & kIoKeep);
 	const uint32_t l0 = round_base<F>(t);
 	// workgroups hold exactly G::THREADS threads, or 64 when a one-tile
 	// size needs fewer
@@ -496,7 +507,7 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	// ---- operands ----
 	// the first loads of an inverse transform may form a pointwise product
 	constexpr bool PRODUCT = INV && FIRST_ROUND && G::P != 1;
-	if (FIRST_ROUND && !RD::DIRECT) {
+	if (FIRST_ROUND && !RD::DIRECT && !PRE) {
 		// lane-contiguous global loads staged through LDS
 		if (active) {
 			if (PRODUCT && src2) {
@@ -518,7 +529,9 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		exchange_sync<!LOCAL>();
 	}
 	if (active) {
-		if (FIRST_ROUND && RD::DIRECT && PRODUCT && src2) {
+		if constexpr (PRE) {
+			// operands already in v
+		} else if (FIRST_ROUND && RD::DIRECT && PRODUCT && src2) {
 #pragma unroll
 			for (int e = 0; e < 8; e++) {
 				const uint32_t gi = tile_addr<G>(l0 + (e << F), jbase);
@@ -662,7 +675,9 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	}
 
 	// ---- results ----
-	if constexpr (LAST_ROUND && RD::DIRECT) {
+	if constexpr (KEEP) {
+		// results stay in v
+	} else if constexpr (LAST_ROUND && RD::DIRECT) {
 		if (active) {
 			// store addresses formed here, not held from the kernel's start
 			const uint32_t ls = round_base<F>(opaque32_here(t));
@@ -699,15 +714,24 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	}
 }
 
+template <class G, bool INV, int LOGN, bool TRAIL, int MC, int IO, int... R>
+__device__ __forceinline__ void run_rounds_v(std::integer_sequence<int, R...>,
+		uint64_t v[8], uint32_t t, const uint64_t *src, const uint64_t *src2,
+		uint64_t *dst, uint64_t *lds, uint32_t jbase, uint32_t gbase,
+		const ulonglong2 *tw, const NttConsts &c, bool final) {
+	ulonglong2 w[7];
+	(run_round<G, INV, R, LOGN, TRAIL, MC, IO>(v, w, t, src, src2, dst, lds,
+			jbase, gbase, tw, c, final), ...);
+}
+
 template <class G, bool INV, int LOGN, bool TRAIL, int MC, int... R>
-__device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...>,
+__device__ __forceinline__ void run_rounds(std::integer_sequence<int, R...> seq,
 		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
 		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
 		const NttConsts &c, bool final) {
 	uint64_t v[8];
-	ulonglong2 w[7];
-	(run_round<G, INV, R, LOGN, TRAIL, MC>(v, w, t, src, src2, dst, lds, jbase,
-			gbase, tw, c, final), ...);
+	run_rounds_v<G, INV, LOGN, TRAIL, MC, 0>(seq, v, t, src, src2, dst, lds,
+			jbase, gbase, tw, c, final);
 }
 
 template <int LOGN, int PASS, int TB, bool INV, int MC>
@@ -803,6 +827,78 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 	}
 }
 
+// The middle of a polynomial product a*b fused into one kernel: a's
+// forward rounds (twiddles tw2) with canonical results kept in registers,
+// the pointwise product with b's spectrum (in2) at the same words, then the
+// inverse rounds (twiddles tw) from those registers.  It saves storing a's
+// spectrum and loading it back.  One-tile sizes run the whole product, one
+// workgroup per task (blockIdx.x); two-pass sizes run the row pass of both
+// transforms, one workgroup per (tile, task) = (blockIdx.x, blockIdx.y), on
+// input whose forward column pass is done, and the inverse column pass
+// follows.  Possible when the forward's last round and the inverse's first
+// both hold their field at bit 0: thread t then holds words 8t..8t+7 in
+// both, and both are wave-local, so a wave fence orders the forward's LDS
+// reads before the inverse's first exchange writes.
+// Two-pass sizes only with rows of 512 words (three rounds): longer rows
+// (n = 2^19, 2^20) spill the product's registers at eight waves per SIMD
+// and measured no faster than the separate kernels.
+template <int LOGN, int TB>
+constexpr bool pm_fusable() {
+	using G = Geo<LOGN, LOGN <= TB ? 0 : 2, TB>;
+	return (LOGN <= TB || G::K2 == 9) && Round<G, false, G::ROUNDS - 1>::F == 0 &&
+		Round<G, true, 0>::F == 0;
+}
+
+// Words i, i + 1 of p (i even).
+__device__ __forceinline__ u64x2 gload2(const uint64_t *p, uint32_t i) {
+	return *(gcu128 *) ((gcchar *) p + (i << 3));
+}
+
+// one-tile sizes leave the register budget to the compiler, as k_ntt_fused
+constexpr int pm_waves(int logn, int tb) {
+	return logn <= tb ? 1 : pass_waves(logn, 2, tb);
+}
+
+template <int LOGN, int TB, int MC>
+__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
+__attribute__((amdgpu_waves_per_eu(pm_waves(LOGN, TB)))) void k_ntt_pm(
+		NttTask uniform, const NttTask *tasks, uint32_t) {
+	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
+	if constexpr (pm_fusable<LOGN, TB>()) {
+		constexpr int PASS = LOGN <= TB ? 0 : 2;
+		using G = Geo<LOGN, PASS, TB>;
+		const uint32_t task = PASS == 0 ? blockIdx.x : blockIdx.y;
+		NttTask k;
+		if (tasks) {
+			k = tasks[task];
+		} else {
+			k = uniform;
+			const uint64_t off = (uint64_t) task << LOGN;
+			k.in += off;
+			k.out += off;
+			k.in2 += off;
+		}
+		const uint32_t t = threadIdx.x;
+		const uint32_t jbase = PASS == 0 ? 0 : blockIdx.x << TB;
+		const auto seq = std::make_integer_sequence<int, G::ROUNDS>();
+		uint64_t v[8];
+		run_rounds_v<G, false, LOGN, false, MC, kIoKeep>(seq, v, t, k.in,
+				nullptr, k.out, lds, jbase, jbase, k.tw2, k.c, true);
+		if (G::THREADS >= 64 || t < (uint32_t) G::THREADS) {
+			const uint32_t l0 = tile_addr<G>(round_base<0>(t), jbase);
+#pragma unroll
+			for (int e = 0; e < 8; e += 2) {
+				const u64x2 y = gload2(k.in2, l0 + e);
+				v[e] = spectra_product(v[e], y.x, k.c);
+				v[e + 1] = spectra_product(v[e + 1], y.y, k.c);
+			}
+		}
+		wave_sync();
+		run_rounds_v<G, true, LOGN, false, MC, kIoPreloaded>(seq, v, t, nullptr,
+				nullptr, k.out, lds, jbase, jbase, k.tw, k.c, PASS == 0);
+	}
+}
+
 // Kernel table for tile size TB, entry i serving log2 n = i + 3.
 template <int TB, int... L>
 constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
@@ -826,6 +922,10 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 		{k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 0>,
 			k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 1>,
 			k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 2>},
+		{k_ntt_pm<L + kMinFusedLogn, TB, 0>,
+			k_ntt_pm<L + kMinFusedLogn, TB, 1>,
+			k_ntt_pm<L + kMinFusedLogn, TB, 2>},
+		pm_fusable<L + kMinFusedLogn, TB>(),
 		FusedShape<L + kMinFusedLogn, TB>::THREADS,
 		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0)}...};
 }
