@@ -683,6 +683,9 @@ def main():
                          "a profiled run uses the kernels an unprofiled run "
                          "picks (kernel tracing serialises dispatches and can "
                          "change the tuner's ranking)")
+    ap.add_argument("--polymul-only", action="store_true",
+                    help="only run the polymul leg (after tuning) and print "
+                         "its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 copies of 256 MiB (elemgtadd with bound "
                          "2^64-1) to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -710,6 +713,9 @@ def main():
     if args.tile_bits:
         ctx.set_tile_bits(args.logn, args.tile_bits)
         ctx.set_pass_mode(args.logn, 2)
+    if args.polymul_only:
+        print(json.dumps({"polymul": polymul_leg(V, ctx, ev)}))
+        return
     tuned = {l: ctx.tile_bits(l) for l in range(12, 18)}
     modes = {l: ctx.pass_mode(l) for l in range(12, 18)}
     t_tune = time.perf_counter() - t_tune
