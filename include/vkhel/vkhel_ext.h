@@ -142,8 +142,8 @@ int vkhel_ext_autotune_load(struct vkhel_ctx *, const char *path);
 
 /* ---- batched transforms ----
  * A batch is a list of jobs executed by one launch sequence on the context
- * stream (captured in a hipGraph and replayed when the job list is
- * unchanged).  A job works on an RNS polynomial: `nlimbs` limbs of n words
+ * stream (planned once while the job list is unchanged; replayed from a
+ * captured hipGraph when it has more than a few launches).  A job works on an RNS polynomial: `nlimbs` limbs of n words
  * stored back to back in one vector (limb i at word offset i*n), limb i
  * reduced modulo tables[i]->q.  Every tables[i] of one job has the same n.
  * Vectors must hold at least nlimbs*n words.  Inputs of NTT jobs follow the

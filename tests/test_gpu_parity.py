@@ -358,6 +358,36 @@ def test_polymul_rns_vs_oracle(ctx, logn, mode):
             t.close()
 
 
+def test_batch_direct_launch_replays(ctx):
+    """A batch of few launches (forward transforms of one size: two pass
+    kernels) is enqueued directly rather than replayed from a graph: each
+    launch reads the inputs as they are at that launch."""
+    n = 1 << 16
+    qs = O.primes_ntt(1 << 50, 2 * n, 2)
+    tabs = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+    xs = [O.fill_uniform(n, qs[i % 2], 70 + i) for i in range(4)]
+    ins = [vec(ctx, x) for x in xs]
+    outs = [ctx.vector(n) for _ in xs]
+    batch = V.Batch(ctx)
+    for i, (a, r) in enumerate(zip(ins, outs)):
+        batch.add(V.JOB_NTT_FWD, a, None, r, [tabs[i % 2]])
+    for rep in range(3):
+        if rep == 2:
+            # new inputs, same job list: the next launch transforms them
+            xs = [O.fill_uniform(n, qs[i % 2], 170 + i) for i in range(4)]
+            for a, x in zip(ins, xs):
+                a.write(x)
+        for r in outs:
+            r.write(np.zeros(n, dtype=np.uint64))
+        batch.run()
+        for i, r in enumerate(outs):
+            assert np.array_equal(r.read(), O.ntt_forward(xs[i], qs[i % 2],
+                                                          tabs[i % 2].w)), (rep, i)
+    batch.close()
+    for x in ins + outs + tabs:
+        x.close()
+
+
 @pytest.mark.parametrize("tb", [11, 12, 13])
 def test_fused_polymul_tiles(ctx, tb):
     """Polymul jobs run a's forward, the product and the inverse in one
