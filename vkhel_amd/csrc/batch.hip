@@ -441,11 +441,23 @@ static int launch_pointwise(const std::vector<PointTask> &h, const PointTask *d,
 	return VKHEL_OK;
 }
 
-static int launch_polymul_segments(const std::vector<Segment> &segs,
-		const NttTask *dev, bool column_pass, hipStream_t s) {
-	for (const Segment &g : segs) {
-		const int rc = launch_polymul_tasks(dev + g.off, g.count, g.logn, g.tb,
-				g.mc, column_pass, s);
+// The fused polymul segments; a two-pass size's column-pass tasks form a
+// segment of the same (log2 n, class) key, with the same jobs in the same
+// order (both lists are filled in job order and sorted stably by that key).
+static int launch_polymul_segments(const TaskSet &ts, const DeviceTasks &dt,
+		hipStream_t s) {
+	for (const Segment &g : ts.seg_pm) {
+		const NttTask *cols = nullptr;
+		for (const Segment &c : ts.seg_pm_col) {
+			if (c.logn == g.logn && c.mc == g.mc) {
+				if (c.count != g.count)
+					return fail(VKHEL_ERR_INVALID, "polymul plan: column segment "
+							"of %u tasks for %u products", c.count, g.count);
+				cols = dt.pm_col + c.off;
+			}
+		}
+		const int rc = launch_polymul_tasks(cols, dt.pm + g.off, g.count, g.logn,
+				g.tb, g.mc, s);
 		if (rc != VKHEL_OK)
 			return rc;
 	}
@@ -456,9 +468,7 @@ int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
 		const DeviceTasks &dt, hipStream_t s) {
 	int rc = launch_segments(ctx, ts.seg_fwd, ts.fwd, dt.fwd, false, s);
 	if (rc == VKHEL_OK)
-		rc = launch_polymul_segments(ts.seg_pm_col, dt.pm_col, true, s);
-	if (rc == VKHEL_OK)
-		rc = launch_polymul_segments(ts.seg_pm, dt.pm, false, s);
+		rc = launch_polymul_segments(ts, dt, s);
 	if (rc == VKHEL_OK)
 		rc = launch_pointwise(ts.pt, dt.pt, ts.pt_maxlen, s);
 	if (rc == VKHEL_OK)

@@ -179,13 +179,13 @@ int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
 // run on for log2 n on this context, 0 when that size and tile do not allow
 // them.
 int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn);
-// Launches one stage of fused polynomial products on 2^tile_bits tiles,
-// tasks {in, out, tw = inverse twiddles, tw2 = forward twiddles, in2 = b's
-// spectrum}.  column_pass: the forward column pass of a (in -> out) of a
-// two-pass size; otherwise the fused forward rows, product and inverse rows
-// (reading in) and, for two-pass sizes, the inverse column pass (out).
-int launch_polymul_tasks(const NttTask *dev_tasks, uint32_t ntasks,
-		uint32_t logn, int tile_bits, int mc, bool column_pass, hipStream_t st);
+// Launches `ntasks` fused polynomial products on 2^tile_bits tiles:
+// pm_tasks {in, out, tw = inverse twiddles, tw2 = forward twiddles, in2 =
+// b's spectrum}.  Two-pass sizes also take col_tasks, task i the forward
+// column pass of pm task i's a (in -> out, forward twiddles), and read a
+// from out in the fused kernel; one-tile sizes take col_tasks null.
+int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
+		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st);
 // Tile size (log2 words) used for log2 n on this context.
 int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn);
 bool ntt_fused_supported(uint32_t logn);

@@ -51,8 +51,9 @@ struct Segment {
 struct TaskSet {
 	std::vector<NttTask> fwd, inv;
 	std::vector<Segment> seg_fwd, seg_inv;
-	// fused polynomial products: the forward column passes of a (two-pass
-	// sizes), then the fused middle (launch_polymul_tasks)
+	// fused polynomial products (launch_polymul_tasks): the forward column
+	// passes of a (two-pass sizes only, task i for product i of the same
+	// key) and the fused products
 	std::vector<NttTask> pm_col, pm;
 	std::vector<Segment> seg_pm_col, seg_pm;
 	std::vector<PointTask> pt, post;  // before / after the inverse phase

@@ -19,6 +19,7 @@
 // equal the reference's stage-by-stage output bit for bit: every butterfly
 // is exact mod q, so only the order of independent butterflies changes.
 // Sizes above 2^20 and the RADIX2 algorithm use the radix-2 kernels below.
+#include <algorithm>
 #include <array>
 #include <utility>
 
@@ -182,6 +183,16 @@ int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn) {
 	return tb ? tb : default_tile_bits(logn);
 }
 
+// Split two-pass transforms run in chunks of at most 2^24 words (128 MiB)
+// of polynomials, both passes of one chunk before the next: the second pass
+// then reads what the first just wrote while it is still held in the
+// 256 MB last-level cache.  Measured on MI355X (forward, 2^16 words,
+// 2^12 tiles): 512 polynomials 0.2135 -> 0.2080 ms in two chunks
+// (interleaved runs in one process); 2048 polynomials 1.047 -> 0.952 ms in
+// four to eight.  Chunks also keep the grid's y dimension within its limit
+// of 65535.
+constexpr uint64_t kChunkWords = 1ull << 24;
+
 // Launches `ntasks` transforms of one size and direction: either through
 // the device task array `dev_tasks` or, when it is null, `uniform` and the
 // ntasks - 1 polynomials contiguous after it.  Split mode runs the two
@@ -191,11 +202,27 @@ static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
 		int mc, hipStream_t st) {
 	const FusedEntry &e = fused_entry(logn, tile_bits);
 	if (split && e.tiles > 1) {
-		const dim3 grid(e.tiles, ntasks);
-		hipLaunchKernelGGL(inverse ? e.inv_b[mc] : e.fwd_a[mc], grid,
-				dim3(e.threads), 0, st, uniform, dev_tasks, 0u);
-		hipLaunchKernelGGL(inverse ? e.inv_a[mc] : e.fwd_b[mc], grid,
-				dim3(e.threads), 0, st, uniform, dev_tasks, 1u);
+		const uint32_t chunk = (uint32_t) std::max<uint64_t>(1,
+				std::min<uint64_t>(kChunkWords >> logn, 65535));
+		for (uint32_t first = 0; first < ntasks; first += chunk) {
+			const uint32_t count = std::min(chunk, ntasks - first);
+			NttTask u = uniform;
+			const NttTask *d = dev_tasks;
+			if (d) {
+				d += first;
+			} else {
+				const uint64_t off = (uint64_t) first << logn;
+				u.in += off;
+				u.out += off;
+				if (u.in2)
+					u.in2 += off;
+			}
+			const dim3 grid(e.tiles, count);
+			hipLaunchKernelGGL(inverse ? e.inv_b[mc] : e.fwd_a[mc], grid,
+					dim3(e.threads), 0, st, u, d, 0u);
+			hipLaunchKernelGGL(inverse ? e.inv_a[mc] : e.fwd_b[mc], grid,
+					dim3(e.threads), 0, st, u, d, 1u);
+		}
 	} else {
 		hipLaunchKernelGGL(inverse ? e.inv[mc] : e.fwd[mc],
 				dim3(ntasks), dim3(e.threads), 0, st, uniform, dev_tasks);
@@ -231,26 +258,33 @@ int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn) {
 	return fused_entry(logn, tb).pm_ok ? tb : 0;
 }
 
-int launch_polymul_tasks(const NttTask *dev_tasks, uint32_t ntasks,
-		uint32_t logn, int tile_bits, int mc, bool column_pass, hipStream_t st) {
+int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
+		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st) {
 	if (ntasks == 0)
 		return VKHEL_OK;
 	const FusedEntry &e = fused_entry(logn, tile_bits);
-	if (!fused_supported(logn) || !e.pm_ok || (column_pass && e.tiles == 1))
+	if (!fused_supported(logn) || !e.pm_ok || ((e.tiles > 1) != (col_tasks != nullptr)))
 		return fail(VKHEL_ERR_INVALID,
 				"fused polymul: unsupported log2 n = %u on 2^%d tiles", logn,
 				tile_bits);
 	const NttTask none{};
-	const dim3 grid = e.tiles > 1 ? dim3(e.tiles, ntasks) : dim3(ntasks);
-	if (column_pass) {
-		hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, none,
-				dev_tasks, 0u);
+	if (e.tiles == 1) {
+		hipLaunchKernelGGL(e.pm[mc], dim3(ntasks), dim3(e.threads), 0, st, none,
+				pm_tasks, 0u);
 	} else {
-		hipLaunchKernelGGL(e.pm[mc], grid, dim3(e.threads), 0, st, none,
-				dev_tasks, 0u);
-		if (e.tiles > 1)
+		// chunked as the split transforms (kChunkWords): the row kernel
+		// then reads a's column pass output while it is still cached
+		const uint32_t chunk = (uint32_t) std::max<uint64_t>(1,
+				std::min<uint64_t>(kChunkWords >> logn, 65535));
+		for (uint32_t first = 0; first < ntasks; first += chunk) {
+			const dim3 grid(e.tiles, std::min(chunk, ntasks - first));
+			hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, none,
+					col_tasks + first, 0u);
+			hipLaunchKernelGGL(e.pm[mc], grid, dim3(e.threads), 0, st, none,
+					pm_tasks + first, 0u);
 			hipLaunchKernelGGL(e.inv_a[mc], grid, dim3(e.threads), 0, st, none,
-					dev_tasks, 1u);
+					pm_tasks + first, 1u);
+		}
 	}
 	return hip_ok(hipGetLastError(), "fused polymul launch") ? VKHEL_OK
 		: VKHEL_ERR_DEVICE;
