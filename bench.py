@@ -478,6 +478,50 @@ def intt_center_leg(V, ctx, ev, logn=16, primes=8, per_prime=64, p=65537,
     return res
 
 
+def rescale_ntt_leg(V, ctx, ev, logn=16, limbs=8, polys=64, reps=5):
+    """RNS rescaling of NTT-form polynomials (drop the last of 8 limbs <
+    2^50 at n = 2^16, 64 polynomials): one RESCALE_NTT job per polynomial
+    (INTT of the dropped limb, its lift into the other limbs, their NTTs,
+    one subtract-and-scale pass) against the textbook composition as
+    three batches (INTT of every limb, coefficient-form RESCALE, NTT of the
+    remaining limbs)."""
+    n = 1 << logn
+    qs = primes_ntt(1 << 50, 2 * n, limbs)
+    tabs = [V.NttTables(n, q, root_2n(q, n)) for q in qs]
+    words = limbs * n
+    A, R, C, S, T = [], [], [], [], []
+    for k in range(polys):
+        a = ctx.vector(words, False)
+        for i, q in enumerate(qs):
+            a.fill_uniform(q, 13 * k + i, i * n, n)
+        A.append(a)
+        R.append(ctx.vector(words - n))
+        C.append(ctx.vector(words))
+        S.append(ctx.vector(words - n))
+        T.append(ctx.vector(words - n))
+    fused, inv, resc, fwd = V.Batch(ctx), V.Batch(ctx), V.Batch(ctx), V.Batch(ctx)
+    for a, r, c, s_, t in zip(A, R, C, S, T):
+        fused.add_rns(V.JOB_RESCALE_NTT, a, r, tabs)
+        inv.add(V.JOB_NTT_INV, a, None, c, tabs)
+        resc.add_rns(V.JOB_RESCALE, c, s_, tabs)
+        fwd.add(V.JOB_NTT_FWD, s_, None, t, tabs[:-1])
+    f_ms = timed_launches(ctx, fused, reps, ev)
+    three_ms = sum(timed_launches(ctx, b, reps, ev) for b in (inv, resc, fwd))
+    same = all(np.array_equal(r.read(), t.read()) for r, t in zip(R[:4], T[:4]))
+    res = {"workload": f"rescale by the last of {limbs} primes <2^50 in NTT form, "
+                       f"{polys} polys, n=2^{logn}",
+           "fused_ms": round(f_ms, 4), "three_batches_ms": round(three_ms, 4),
+           "speedup": round(three_ms / f_ms, 3),
+           # read a (limbs x n words), write the result ((limbs - 1) x n)
+           "fused_alg_GBps": round(8 * (2 * limbs - 1) * n * polys / f_ms / 1e6, 1),
+           "results_equal": bool(same)}
+    for v in A + R + C + S + T:
+        v.close()
+    for b in (fused, inv, resc, fwd):
+        b.close()
+    return res
+
+
 def pcie_inclusive(V, ctx, logn=16, primes=8, per_prime=64, chunks=8,
                    per_call_sample=32):
     """Host-resident inputs and outputs (the headline batch, n = 2^16,
@@ -747,6 +791,7 @@ def main():
             b.close()
         extra["polymul"] = polymul_leg(V, ctx, ev)
         extra["intt_center"] = intt_center_leg(V, ctx, ev)
+        extra["rescale_ntt"] = rescale_ntt_leg(V, ctx, ev)
         extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
         extra["plain_polymul_n4096"] = plain_polymul_n4096(V, ctx)
         extra["pcie_inclusive"] = pcie_inclusive(V, ctx)

@@ -174,6 +174,14 @@ enum vkhel_job_op {
 	 * canonical; res must not alias a.  Runs after every transform of the
 	 * batch, so it may consume an INTT of the same batch. */
 	VKHEL_JOB_RESCALE = 9,
+	/* the same rescaling on a polynomial in NTT form, result in NTT form:
+	 * res_i = (a_i - NTT_i([INTT_last(a_last)])) * q_last^-1 mod q_i for
+	 * i < nlimbs - 1, [.] the centered lift into Z_q_i; that is NTT_i of
+	 * VKHEL_JOB_RESCALE applied to the INTTs of the limbs.  Runs as an
+	 * inverse transform of the last limb, the lift into every other limb,
+	 * their forward transforms and one pointwise pass, after every other
+	 * transform of the batch.  Same checks as VKHEL_JOB_RESCALE. */
+	VKHEL_JOB_RESCALE_NTT = 10,
 };
 
 struct vkhel_batch;
@@ -185,11 +193,13 @@ int vkhel_batch_add(struct vkhel_batch *, int op,
 		const struct vkhel_vector *a, const struct vkhel_vector *b,
 		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,
 		uint32_t nlimbs);
-/* Adds an RNS elementwise job (VKHEL_JOB_CENTER / _GTADD / _GTSUB): every
+/* Adds an RNS job (VKHEL_JOB_CENTER / _GTADD / _GTSUB, elementwise: every
  * limb of every such job in the batch runs in the batch's one pointwise
- * launch.  CENTER is the centered lift of limb i (mod q_i) reduced mod p =
+ * launch; _INTT_CENTER, _RESCALE, _RESCALE_NTT as documented above).
+ * CENTER is the centered lift of limb i (mod q_i) reduced mod p =
  * params[0]; GTADD / GTSUB take (bound, diff) / (bound, diff, mod) per
- * limb.  `params` is copied. */
+ * limb; the rescaling ops take none (params may be NULL).  `params` is
+ * copied. */
 int vkhel_batch_add_rns(struct vkhel_batch *, int op,
 		const struct vkhel_vector *a, struct vkhel_vector *res,
 		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs,

@@ -939,3 +939,55 @@ def test_rns_rescale_matches_big_integer_division(ctx, L):
         v.close()
     for t in tabs:
         t.close()
+
+
+@pytest.mark.parametrize("logn,bits", [(4, (50, 61)), (10, (50, 61, 55, 40)),
+                                       (14, (59, 50, 61)), (16, (50, 55, 60, 59))])
+def test_rns_rescale_ntt_form(ctx, logn, bits):
+    """Rescaling in NTT form: the INTT of each limb, the big-integer rounded
+    division by the last modulus, the NTT of each remaining limb, against one
+    RESCALE_NTT job; one- and two-pass sizes, every modulus class."""
+    n = 1 << logn
+    qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in bits]
+    ws = [O.root_2n(q, n) for q in qs]
+    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+    L = len(qs)
+    spec = [O.fill_uniform(n, q, 300 + i) for i, q in enumerate(qs)]
+    coeffs = [O.ntt_inverse(s, q, w) for s, q, w in zip(spec, qs, ws)]
+    # the dropped limb's centered-lift boundary values, set in coefficient form
+    coeffs[-1][:4] = [0, qs[-1] // 2, qs[-1] // 2 + 1, qs[-1] - 1]
+    spec[-1] = O.ntt_forward(coeffs[-1], qs[-1], ws[-1])
+    if n > 1 << 12:  # the big-integer oracle on a sample of coefficients
+        keep = np.r_[0:64, n - 64:n]
+    else:
+        keep = np.arange(n)
+    want_c = _rescale_oracle([c[keep] for c in coeffs], qs)
+    a = vec(ctx, np.concatenate(spec))
+    r = ctx.vector((L - 1) * n)
+    b = V.Batch(ctx)
+    b.add_rns(V.JOB_RESCALE_NTT, a, r, tabs)
+    b.run()
+    got = r.read()
+    for i in range(L - 1):
+        got_c = O.ntt_inverse(got[i * n:(i + 1) * n], qs[i], ws[i])
+        assert np.array_equal(got_c[keep], want_c[i]), i
+    # the whole result against the coefficient-form job on the same limbs
+    c = vec(ctx, np.concatenate(coeffs))
+    r2 = ctx.vector((L - 1) * n)
+    b2 = V.Batch(ctx)
+    b2.add_rns(V.JOB_RESCALE, c, r2, tabs)
+    b2.run()
+    flat = r2.read()
+    for i in range(L - 1):
+        assert np.array_equal(got[i * n:(i + 1) * n],
+                              O.ntt_forward(flat[i * n:(i + 1) * n], qs[i], ws[i])), i
+    with pytest.raises(V.VkhelError):
+        b2.add_rns(V.JOB_RESCALE_NTT, a, a, tabs)
+    with pytest.raises(V.VkhelError):
+        b2.add_rns(V.JOB_RESCALE_NTT, a, r, tabs[:1])
+    for o in (b, b2):
+        o.close()
+    for v in (a, r, c, r2):
+        v.close()
+    for t in tabs:
+        t.close()

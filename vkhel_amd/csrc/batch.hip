@@ -1,8 +1,8 @@
 // Batched execution of many RNS polynomial jobs (vkhel_ext.h).  The
 // reference has no batch API: a caller would issue one synchronous
 // vkhel_vector_* call per polynomial (src/vector.c:513-657), each paying
-// log2(n) fenced submits.  Here a job list turns into three device phases,
-// each a handful of launches over every polynomial limb of every job:
+// log2(n) fenced submits.  Here a job list turns into device phases, each
+// a handful of launches over every polynomial limb of every job:
 //
 //   1. forward transforms   (NTT_FWD jobs, both operands of POLYMUL jobs,
 //                            only b where the polymul kernel is fused)
@@ -12,6 +12,9 @@
 //                            the fused kernels do not cover)
 //   3. inverse transforms   (NTT_INV jobs, POLYMUL results; the fused
 //                            inverse multiplies the two spectra as it loads)
+//   4. pointwise            (RESCALE; RESCALE_NTT's lift of the dropped limb)
+//   5. forward transforms   (RESCALE_NTT's lifted limbs)
+//   6. pointwise            (RESCALE_NTT's subtraction and scaling)
 //
 // Transform tasks are grouped by size: one launch per distinct n, each
 // workgroup taking one polynomial limb.  The batch API captures the launch
@@ -86,11 +89,14 @@ void DeviceTasks::release() {
 	(void) hipFree(inv);
 	(void) hipFree(pm_col);
 	(void) hipFree(pm);
+	(void) hipFree(fwd2);
 	(void) hipFree(pt);
 	(void) hipFree(post);
-	fwd = inv = pm_col = pm = nullptr;
-	pt = post = nullptr;
-	cap_fwd = cap_inv = cap_pm_col = cap_pm = cap_pt = cap_post = 0;
+	(void) hipFree(post2);
+	fwd = inv = pm_col = pm = fwd2 = nullptr;
+	pt = post = post2 = nullptr;
+	cap_fwd = cap_inv = cap_pm_col = cap_pm = cap_fwd2 = 0;
+	cap_pt = cap_post = cap_post2 = 0;
 }
 
 int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
@@ -130,12 +136,12 @@ static const uint64_t kNoParams = 0;
 int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		vkhel_vector *res, vkhel_ntt_tables *const *tables, uint32_t nlimbs,
 		const uint64_t *params, Job *out) {
-	if (!a || !res || !tables || nlimbs == 0 ||
-			(!params && op != VKHEL_JOB_RESCALE))
+	const bool rescale = op == VKHEL_JOB_RESCALE || op == VKHEL_JOB_RESCALE_NTT;
+	if (!a || !res || !tables || nlimbs == 0 || (!params && !rescale))
 		return fail(VKHEL_ERR_INVALID, "rns job: null argument");
-	if (op < VKHEL_JOB_CENTER || op > VKHEL_JOB_RESCALE)
+	if (op < VKHEL_JOB_CENTER || op > VKHEL_JOB_RESCALE_NTT)
 		return fail(VKHEL_ERR_INVALID, "rns job: unknown op %d", op);
-	if (op == VKHEL_JOB_RESCALE && !params)
+	if (rescale && !params)
 		params = &kNoParams;
 	if (a->ctx != ctx || res->ctx != ctx)
 		return fail(VKHEL_ERR_INVALID, "rns job: vector from another context");
@@ -145,14 +151,13 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 					"rns job: limb tables must all exist and share n");
 	const uint64_t words = (uint64_t) nlimbs * tables[0]->n;
 	// rescaling drops the last limb from the result
-	const uint64_t res_words = op == VKHEL_JOB_RESCALE
-		? words - tables[0]->n : words;
+	const uint64_t res_words = rescale ? words - tables[0]->n : words;
 	if (a->length < words || res->length < res_words)
 		return fail(VKHEL_ERR_INVALID,
 				"rns job: vectors need %llu and %llu words (%u limbs x n)",
 				(unsigned long long) words, (unsigned long long) res_words,
 				nlimbs);
-	if (op == VKHEL_JOB_RESCALE) {
+	if (rescale) {
 		if (nlimbs < 2)
 			return fail(VKHEL_ERR_INVALID, "rns job: rescale needs 2+ limbs");
 		const uint64_t ql = tables[nlimbs - 1]->q;
@@ -166,7 +171,7 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 					"rns job: rescale reads the dropped limb; res must not be a");
 	}
 	const bool center = op == VKHEL_JOB_CENTER || op == VKHEL_JOB_INTT_CENTER;
-	const size_t np = op == VKHEL_JOB_RESCALE ? 0 : center ? 1 :
+	const size_t np = rescale ? 0 : center ? 1 :
 		(op == VKHEL_JOB_GTADD ? 2 : 3) * (size_t) nlimbs;
 	if (center && params[0] < 2)
 		return fail(VKHEL_ERR_INVALID, "rns job: center needs p >= 2");
@@ -188,6 +193,8 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 }
 
 uint64_t job_scratch_words(const Job &j) {
+	if (j.op == VKHEL_JOB_RESCALE_NTT)
+		return j.tables[0]->n;
 	return j.op == VKHEL_JOB_POLYMUL ? j.tables.size() * j.tables[0]->n : 0;
 }
 
@@ -205,6 +212,7 @@ uint64_t job_transform_coeffs(const Job &j) {
 	case VKHEL_JOB_NTT_FWD:
 	case VKHEL_JOB_NTT_INV:
 	case VKHEL_JOB_INTT_CENTER:
+	case VKHEL_JOB_RESCALE_NTT:  // the dropped limb's INTT, the others' NTTs
 		return words;
 	default:
 		return 0;
@@ -238,7 +246,13 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 		const uint64_t n = j.tables[0]->n;
 		// device twiddles only for jobs that transform
 		const bool transforms = j.op <= VKHEL_JOB_POLYMUL ||
-			j.op == VKHEL_JOB_INTT_CENTER;
+			j.op == VKHEL_JOB_INTT_CENTER || j.op == VKHEL_JOB_RESCALE_NTT;
+		// RESCALE_NTT: the dropped limb's inverse transform
+		uint64_t *dropped = nullptr;
+		if (j.op == VKHEL_JOB_RESCALE_NTT) {
+			dropped = sp;
+			sp += n;
+		}
 		for (size_t l = 0; l < j.tables.size(); l++) {
 			const DevTables *t = transforms ? device_tables(ctx, j.tables[l])
 				: nullptr;
@@ -267,6 +281,32 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 					kPtRescale, w, shoup63_factor(w, q)};
 				ts.post.push_back(p);
 				ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
+				break;
+			}
+			case VKHEL_JOB_RESCALE_NTT: {
+				// INTT of the last limb; per other limb l: its centered lift
+				// into Z_q_l (elemgtsub with bound q_last / 2, diff q_last),
+				// the forward transform of that in place in res limb l, then
+				// (a_l - res_l) * q_last^-1 (the rescale op with a bound no
+				// word exceeds: no lift)
+				const size_t last = j.tables.size() - 1;
+				if (l == last) {
+					ts.inv.push_back(make_task(*t, j.a->dev + off, dropped, true));
+					break;
+				}
+				const uint64_t ql = j.tables[last]->q;
+				PointTask lift{dropped, nullptr, j.res->dev + off, n, q, barrett_mu(q),
+					ql / 2, ql, (uint32_t) (bitlen(q) - 2), kPtGtSub};
+				ts.post.push_back(lift);
+				ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
+				ts.fwd2.push_back(make_task(*t, j.res->dev + off, j.res->dev + off,
+						false));
+				const uint64_t w = invmod_euclid(ql % q, q);
+				PointTask p{j.a->dev + off, j.res->dev + off, j.res->dev + off, n, q,
+					barrett_mu(q), ~0ull, 0, (uint32_t) (bitlen(q) - 2), kPtRescale, w,
+					shoup63_factor(w, q)};
+				ts.post2.push_back(p);
+				ts.post2_maxlen = std::max<uint64_t>(ts.post2_maxlen, n);
 				break;
 			}
 			case VKHEL_JOB_INTT_CENTER: {
@@ -363,12 +403,13 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 			}
 			}
 			if (j.op >= VKHEL_JOB_MUL && j.op != VKHEL_JOB_INTT_CENTER &&
-					j.op != VKHEL_JOB_RESCALE)
+					j.op != VKHEL_JOB_RESCALE && j.op != VKHEL_JOB_RESCALE_NTT)
 				ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
 		}
 	}
 	ts.seg_fwd = segment(ts.fwd);
 	ts.seg_inv = segment(ts.inv);
+	ts.seg_fwd2 = segment(ts.fwd2);
 	ts.seg_pm_col = segment(ts.pm_col);
 	ts.seg_pm = segment(ts.pm);
 	for (auto *segs : {&ts.seg_pm_col, &ts.seg_pm}) {
@@ -401,6 +442,8 @@ int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s) {
 			!upload_array(ts.inv, &dt.inv, &dt.cap_inv, s) ||
 			!upload_array(ts.pm_col, &dt.pm_col, &dt.cap_pm_col, s) ||
 			!upload_array(ts.pm, &dt.pm, &dt.cap_pm, s) ||
+			!upload_array(ts.fwd2, &dt.fwd2, &dt.cap_fwd2, s) ||
+			!upload_array(ts.post2, &dt.post2, &dt.cap_post2, s) ||
 			!upload_array(ts.pt, &dt.pt, &dt.cap_pt, s) ||
 			!upload_array(ts.post, &dt.post, &dt.cap_post, s))
 		return vkhel_ext_last_error();
@@ -475,6 +518,10 @@ int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
 		rc = launch_segments(ctx, ts.seg_inv, ts.inv, dt.inv, true, s);
 	if (rc == VKHEL_OK)
 		rc = launch_pointwise(ts.post, dt.post, ts.post_maxlen, s);
+	if (rc == VKHEL_OK)
+		rc = launch_segments(ctx, ts.seg_fwd2, ts.fwd2, dt.fwd2, false, s);
+	if (rc == VKHEL_OK)
+		rc = launch_pointwise(ts.post2, dt.post2, ts.post2_maxlen, s);
 	return rc;
 }
 

@@ -1,8 +1,9 @@
 // Job model shared by the batch API (batch.hip) and the serving engine
 // (serve.hip): a job is one operation on an RNS polynomial (nlimbs limbs of
 // n words, limb i modulo tables[i]->q); a set of jobs is planned into three
-// device phases of tasks (forward transforms, pointwise, inverse), plus a
-// pointwise phase after the inverse for RNS rescaling.
+// device phases of tasks (forward transforms, pointwise, inverse), plus, for
+// RNS rescaling, a pointwise phase after the inverse and (NTT-form
+// rescaling) a second forward phase and pointwise phase after that.
 #pragma once
 
 #include <vector>
@@ -58,14 +59,20 @@ struct TaskSet {
 	std::vector<Segment> seg_pm_col, seg_pm;
 	std::vector<PointTask> pt, post;  // before / after the inverse phase
 	uint64_t pt_maxlen = 0, post_maxlen = 0;
+	// after `post`: forward transforms, then pointwise tasks
+	std::vector<NttTask> fwd2;
+	std::vector<Segment> seg_fwd2;
+	std::vector<PointTask> post2;
+	uint64_t post2_maxlen = 0;
 };
 
 // Device copies of a TaskSet's arrays, grown on demand.
 struct DeviceTasks {
 	NttTask *fwd = nullptr, *inv = nullptr, *pm_col = nullptr, *pm = nullptr;
-	PointTask *pt = nullptr, *post = nullptr;
-	size_t cap_fwd = 0, cap_inv = 0, cap_pm_col = 0, cap_pm = 0;
-	size_t cap_pt = 0, cap_post = 0;
+	NttTask *fwd2 = nullptr;
+	PointTask *pt = nullptr, *post = nullptr, *post2 = nullptr;
+	size_t cap_fwd = 0, cap_inv = 0, cap_pm_col = 0, cap_pm = 0, cap_fwd2 = 0;
+	size_t cap_pt = 0, cap_post = 0, cap_post2 = 0;
 	void release();
 };
 
@@ -74,11 +81,12 @@ int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		const vkhel_vector *b, vkhel_vector *res,
 		vkhel_ntt_tables *const *tables, uint32_t nlimbs, Job *out);
 // Checks an RNS job (VKHEL_JOB_CENTER / _GTADD / _GTSUB / _INTT_CENTER /
-// _RESCALE).
+// _RESCALE / _RESCALE_NTT).
 int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		vkhel_vector *res, vkhel_ntt_tables *const *tables, uint32_t nlimbs,
 		const uint64_t *params, Job *out);
-// Device scratch words a job needs (the spectrum of b for POLYMUL).
+// Device scratch words a job needs (the spectrum of b for POLYMUL, the
+// dropped limb's inverse transform for RESCALE_NTT).
 uint64_t job_scratch_words(const Job &j);
 // Device bytes a job touches (operands, result, scratch): the serving
 // engine's admission cost.
@@ -88,7 +96,7 @@ uint64_t job_transform_coeffs(const Job &j);
 // Plans `jobs` into tasks; POLYMUL scratch is carved from `scratch`.
 int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 		uint64_t *scratch, TaskSet &ts);
-// Copies the task arrays to `dt` (stream-ordered) and enqueues the three
+// Copies the task arrays to `dt` (stream-ordered) and enqueues the
 // phases on `s`.
 int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s);
 int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,

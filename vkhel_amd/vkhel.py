@@ -17,6 +17,7 @@ JOB_NTT_FWD, JOB_NTT_INV, JOB_POLYMUL, JOB_MUL, JOB_ADD = range(5)
 JOB_CENTER, JOB_GTADD, JOB_GTSUB = 5, 6, 7  # RNS elementwise (add_rns)
 JOB_INTT_CENTER = 8  # fused INTT + centered lift (add_rns, params [p])
 JOB_RESCALE = 9  # drop the last limb, rounding (add_rns, no params)
+JOB_RESCALE_NTT = 10  # the same on NTT-form limbs, result in NTT form
 
 
 def _L():
@@ -325,7 +326,8 @@ class Batch:
     def add_rns(self, op, a, res, tables, params=None):
         """vkhel_batch_add_rns: JOB_CENTER or JOB_INTT_CENTER (params [p]),
         JOB_GTADD ([bound_i, diff_i] per limb), JOB_GTSUB
-        ([bound_i, diff_i, mod_i]) or JOB_RESCALE (no params)."""
+        ([bound_i, diff_i, mod_i]) or JOB_RESCALE / JOB_RESCALE_NTT (no
+        params)."""
         tabs = list(tables)
         arr = (ctypes.c_void_p * len(tabs))(*[t.handle for t in tabs])
         prm = None if params is None else np.ascontiguousarray(params, dtype=np.uint64)
