@@ -941,12 +941,15 @@ def test_rns_rescale_matches_big_integer_division(ctx, L):
         t.close()
 
 
-@pytest.mark.parametrize("logn,bits", [(4, (50, 61)), (10, (50, 61, 55, 40)),
-                                       (14, (59, 50, 61)), (16, (50, 55, 60, 59))])
+@pytest.mark.parametrize("logn,bits", [(2, (50, 61)), (4, (50, 61)),
+                                       (10, (50, 61, 55, 40)), (14, (59, 50, 61)),
+                                       (16, (50, 55, 60, 59)), (21, (50, 61))])
 def test_rns_rescale_ntt_form(ctx, logn, bits):
     """Rescaling in NTT form: the INTT of each limb, the big-integer rounded
     division by the last modulus, the NTT of each remaining limb, against one
-    RESCALE_NTT job; one- and two-pass sizes, every modulus class."""
+    RESCALE_NTT job; one- and two-pass sizes, every modulus class, and sizes
+    outside the fused kernels (n = 4, 2^21: lift and scaling as pointwise
+    passes around the radix-2 transforms)."""
     n = 1 << logn
     qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in bits]
     ws = [O.root_2n(q, n) for q in qs]

@@ -285,23 +285,35 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 			}
 			case VKHEL_JOB_RESCALE_NTT: {
 				// INTT of the last limb; per other limb l: its centered lift
-				// into Z_q_l (elemgtsub with bound q_last / 2, diff q_last),
-				// the forward transform of that in place in res limb l, then
-				// (a_l - res_l) * q_last^-1 (the rescale op with a bound no
-				// word exceeds: no lift)
+				// into Z_q_l, the forward transform of that into res limb l,
+				// then (a_l - res_l) * q_last^-1.  The fused forward kernels
+				// apply the lift as they load and the last step as they
+				// store; other sizes run them as pointwise passes (elemgtsub
+				// with bound q_last / 2, diff q_last; the rescale op with a
+				// bound no word exceeds: no lift).
 				const size_t last = j.tables.size() - 1;
 				if (l == last) {
 					ts.inv.push_back(make_task(*t, j.a->dev + off, dropped, true));
 					break;
 				}
 				const uint64_t ql = j.tables[last]->q;
+				const uint64_t w = invmod_euclid(ql % q, q);
+				if (ntt_fused_supported(t->c.logn)) {
+					NttTask f = make_task(*t, dropped, j.res->dev + off, false);
+					f.c.lift_bound = ql / 2;
+					f.c.lift_add = q - ql % q;
+					f.c.scale_w = w;
+					f.c.scale_wp = shoup63_factor(w, q);
+					f.in2 = j.a->dev + off;
+					ts.fwd2.push_back(f);
+					break;
+				}
 				PointTask lift{dropped, nullptr, j.res->dev + off, n, q, barrett_mu(q),
 					ql / 2, ql, (uint32_t) (bitlen(q) - 2), kPtGtSub};
 				ts.post.push_back(lift);
 				ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
 				ts.fwd2.push_back(make_task(*t, j.res->dev + off, j.res->dev + off,
 						false));
-				const uint64_t w = invmod_euclid(ql % q, q);
 				PointTask p{j.a->dev + off, j.res->dev + off, j.res->dev + off, n, q,
 					barrett_mu(q), ~0ull, 0, (uint32_t) (bitlen(q) - 2), kPtRescale, w,
 					shoup63_factor(w, q)};

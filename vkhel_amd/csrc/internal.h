@@ -42,6 +42,15 @@ struct NttConsts {
 	// Barrett constants of post_p
 	uint64_t post_p, post_mu;
 	uint32_t post_shift;
+	// prologue of a forward transform (lift_add 0: none): every input word
+	// x > lift_bound becomes x + lift_add (with lift_add = q - (q' mod q),
+	// the centered lift of a word mod q' into Z_q, left for the first
+	// stage to reduce)
+	uint64_t lift_bound, lift_add;
+	// epilogue of a forward transform (scale_w 0: none): the result y at
+	// word i becomes (in2[i] - y) * scale_w mod q, scale_wp the packed
+	// Shoup companion of scale_w
+	uint64_t scale_w, scale_wp;
 	uint32_t bshift;  // bits - 2
 	uint32_t logn;
 };
@@ -156,9 +165,10 @@ struct NttTask {
 	uint64_t *out;
 	const ulonglong2 *tw;
 	NttConsts c;
-	// Inverse transforms only (fused kernels): when set, the transform's
+	// Fused kernels only.  Inverse transforms: when set, the transform's
 	// input is the pointwise product in[i] * in2[i] mod q of two canonical
 	// spectra, formed as they are loaded (a polynomial product's last step).
+	// Forward transforms: the operand of the c.scale_* epilogue.
 	const uint64_t *in2;
 	// Forward twiddles of a fused polynomial-product task (tw: inverse).
 	const ulonglong2 *tw2;

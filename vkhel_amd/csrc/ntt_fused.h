@@ -187,6 +187,21 @@ __device__ __forceinline__ uint64_t post_center(uint64_t x,
 	return r;
 }
 
+// The forward transform's optional prologue and epilogue (NttConsts
+// lift_* and scale_*), used by NTT-form RNS rescaling: the centered lift
+// of the dropped limb's words into Z_q as they load (non-canonical words
+// take the first stage's exact reduction), and (a - y) * w mod q as the
+// canonical results store.
+__device__ __forceinline__ uint64_t lift_word(uint64_t x, const NttConsts &c) {
+	return x > c.lift_bound ? x + c.lift_add : x;
+}
+
+__device__ __forceinline__ uint64_t scale_word(uint64_t a, uint64_t y,
+		const NttConsts &c) {
+	const uint64_t d = a >= y ? a - y : a + c.q - y;
+	return csub(shoup63(d, c.scale_w, c.scale_wp, 0 - c.q), c.q);
+}
+
 // Inverse butterflies for q < kNoReduceQ with the sums left unreduced
 // inside a round.  x and y lie below bx*q and by*q; C = by*q keeps the
 // difference non-negative, and the product input x + C - y stays below
@@ -507,6 +522,8 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	// ---- operands ----
 	// the first loads of an inverse transform may form a pointwise product
 	constexpr bool PRODUCT = INV && FIRST_ROUND && G::P != 1;
+	// the forward's optional prologue, on the loads of its input
+	constexpr bool LIFT = !INV && FIRST_ROUND && G::P != 2 && !PRE;
 	if (FIRST_ROUND && !RD::DIRECT && !PRE) {
 		// lane-contiguous global loads staged through LDS
 		if (active) {
@@ -519,6 +536,11 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 					lds[Staged<G>::lds_at(t, k)] = spectra_product(gload(src, gi),
 							gload(src2, gi), c);
 				}
+			} else if (LIFT && c.lift_add) {
+#pragma unroll
+				for (int k = 0; k < 8; k++)
+					lds[Staged<G>::lds_at(t, k)] = lift_word(gload(src,
+							Staged<G>::global_at(t, k, jbase)), c);
 			} else {
 #pragma unroll
 				for (int k = 0; k < 8; k++)
@@ -537,6 +559,10 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 				const uint32_t gi = tile_addr<G>(l0 + (e << F), jbase);
 				v[e] = spectra_product(gload(src, gi), gload(src2, gi), c);
 			}
+		} else if (FIRST_ROUND && RD::DIRECT && LIFT && c.lift_add) {
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+				v[e] = lift_word(gload(src, tile_addr<G>(l0 + (e << F), jbase)), c);
 		} else {
 #pragma unroll
 			for (int e = 0; e < 8; e++) {
@@ -678,6 +704,9 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	if constexpr (KEEP) {
 		// results stay in v
 	} else if constexpr (LAST_ROUND && RD::DIRECT) {
+		// (a forward transform's final round holds its field at bit 0 and
+		// always stores through LDS, where the epilogue is applied)
+		static_assert(INV || G::P == 1, "forward final round stores directly");
 		if (active) {
 			// store addresses formed here, not held from the kernel's start
 			const uint32_t ls = round_base<F>(opaque32_here(t));
@@ -697,10 +726,19 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		exchange_sync<!LOCAL>();
 		if (active) {
 			const uint32_t ts = opaque32_here(t);
+			if (!INV && final && c.scale_w) {
 #pragma unroll
-			for (int k = 0; k < 8; k++)
-				gstore(dst, Staged<G>::global_at(ts, k, jbase),
-						lds[Staged<G>::lds_at(ts, k)]);
+				for (int k = 0; k < 8; k++) {
+					const uint32_t gi = Staged<G>::global_at(ts, k, jbase);
+					gstore(dst, gi, scale_word(gload(src2, gi),
+							lds[Staged<G>::lds_at(ts, k)], c));
+				}
+			} else {
+#pragma unroll
+				for (int k = 0; k < 8; k++)
+					gstore(dst, Staged<G>::global_at(ts, k, jbase),
+							lds[Staged<G>::lds_at(ts, k)]);
+			}
 		}
 		if constexpr (TRAIL)
 			__syncthreads();
@@ -780,7 +818,7 @@ __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 	} else if constexpr (!INV) {
 		run_pass<LOGN, 1, TB, INV, MC>(k.in, nullptr, k.out, lds, k.tw, k.c,
 				false);
-		run_pass<LOGN, 2, TB, INV, MC>(k.out, nullptr, k.out, lds, k.tw, k.c,
+		run_pass<LOGN, 2, TB, INV, MC>(k.out, k.in2, k.out, lds, k.tw, k.c,
 				true);
 	} else {
 		run_pass<LOGN, 2, TB, INV, MC>(k.in, k.in2, k.out, lds, k.tw, k.c, false);
@@ -820,9 +858,12 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 		const uint32_t i = blockIdx.x;
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
+		// in2: the product operand of an inverse's first pass, the
+		// epilogue operand of a forward's second
+		const bool with_in2 = INV ? second == 0 : second != 0;
 		run_rounds<G, INV, LOGN, false, MC>(
 				std::make_integer_sequence<int, G::ROUNDS>(), threadIdx.x,
-				second ? k.out : k.in, second ? nullptr : k.in2, k.out, lds, jbase,
+				second ? k.out : k.in, with_in2 ? k.in2 : nullptr, k.out, lds, jbase,
 				gbase, k.tw, k.c, second != 0);
 	}
 }
