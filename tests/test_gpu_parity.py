@@ -994,3 +994,83 @@ def test_rns_rescale_ntt_form(ctx, logn, bits):
         v.close()
     for t in tabs:
         t.close()
+
+
+def test_batch_mixes_every_job_kind(ctx):
+    """Every job kind in one batch (forward, inverse, polymul, mul, add, the
+    RNS elementwise ops, INTT_CENTER and both rescalings) over sizes 2^5,
+    2^12 and 2^15, so every phase runs with several segments and the
+    polymul and rescale scratch is carved for many jobs; each result
+    against the oracle, then again after a replay."""
+    rng = np.random.default_rng(5)
+    batch = V.Batch(ctx)
+    keep, checks = [], []
+    for k in range(30):
+        logn = (5, 12, 15)[k % 3]
+        n = 1 << logn
+        L = 2 + k % 3
+        qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in (50, 59, 61, 55)[:L]]
+        ws = [O.root_2n(q, n) for q in qs]
+        tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+        xa = [O.fill_uniform(n, q, 900 + 10 * k + i) for i, q in enumerate(qs)]
+        xb = [O.fill_uniform(n, q, 950 + 10 * k + i) for i, q in enumerate(qs)]
+        a, b = vec(ctx, np.concatenate(xa)), vec(ctx, np.concatenate(xb))
+        kind = k % 10
+        if kind <= 4:
+            r = ctx.vector(L * n)
+            batch.add(kind, a, b if kind >= 2 else None, r, tabs)
+            fn = [lambda x, y, q, w: O.ntt_forward(x, q, w),
+                  lambda x, y, q, w: O.ntt_inverse(x, q, w),
+                  lambda x, y, q, w: O.polymul_ntt(x, y, q, w),
+                  lambda x, y, q, w: O.elemmul(x, y, q),
+                  lambda x, y, q, w: O.addmul_mod(x, y, 1, q)][kind]
+            want = [fn(x, y, q, w) for x, y, q, w in zip(xa, xb, qs, ws)]
+        elif kind == 5:
+            r = ctx.vector(L * n)
+            batch.add_rns(V.JOB_CENTER, a, r, tabs, [65537])
+            want = [O.elemmod(x, 65537, q) for x, q in zip(xa, qs)]
+        elif kind == 6:
+            r = ctx.vector(L * n)
+            prm = [int(v) for q in qs for v in (q // 3, 12345)]
+            batch.add_rns(V.JOB_GTADD, a, r, tabs, prm)
+            want = [O.elemgtadd(x, q // 3, 12345) for x, q in zip(xa, qs)]
+        elif kind == 7:
+            r = ctx.vector(L * n)
+            batch.add_rns(V.JOB_INTT_CENTER, a, r, tabs, [3])
+            want = [O.elemmod(O.ntt_inverse(x, q, w), 3, q)
+                    for x, q, w in zip(xa, qs, ws)]
+        else:
+            r = ctx.vector((L - 1) * n)
+            coeffs = xa if kind == 8 else [O.ntt_inverse(x, q, w)
+                                           for x, q, w in zip(xa, qs, ws)]
+            keep_i = np.arange(min(n, 64))
+            rc = _rescale_oracle([c[keep_i] for c in coeffs], qs)
+            batch.add_rns(V.JOB_RESCALE if kind == 8 else V.JOB_RESCALE_NTT,
+                          a, r, tabs)
+            want = (kind, keep_i, rc, qs, ws)
+        keep += [a, b, r] + tabs
+        checks.append((k, n, r, want))
+
+    def verify():
+        for k, n, r, want in checks:
+            got = r.read()
+            if isinstance(want, tuple):
+                kind, keep_i, rc, qs, ws = want
+                for i in range(len(qs) - 1):
+                    g = got[i * n:(i + 1) * n]
+                    if kind == 9:
+                        g = O.ntt_inverse(g, qs[i], ws[i])
+                    assert np.array_equal(g[keep_i], rc[i]), (k, i)
+            else:
+                for i, wv in enumerate(want):
+                    assert np.array_equal(got[i * n:(i + 1) * n], wv), (k, i)
+
+    batch.run()
+    verify()
+    for _, _, r, _ in checks:
+        r.write(np.zeros(r.length, dtype=np.uint64))
+    batch.run()
+    verify()
+    batch.close()
+    for x in keep:
+        x.close()
