@@ -268,6 +268,12 @@ uint64_t vkhel_server_submit(struct vkhel_server *, int op,
 		const struct vkhel_vector *a, const struct vkhel_vector *b,
 		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,
 		uint32_t nlimbs);
+/* As vkhel_server_submit for the RNS jobs of vkhel_batch_add_rns
+ * (VKHEL_JOB_CENTER ... VKHEL_JOB_RESCALE_NTT, the same arguments). */
+uint64_t vkhel_server_submit_rns(struct vkhel_server *, int op,
+		const struct vkhel_vector *a, struct vkhel_vector *res,
+		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs,
+		const uint64_t *params);
 /* Waits up to timeout_ns (0: no limit).  Returns the request's status, or
  * VKHEL_ERR_TIMEOUT if it is still pending.  A completed request is
  * released by the wait that returns its status. */

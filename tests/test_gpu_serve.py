@@ -177,3 +177,41 @@ def test_queue_full_cancel_and_timeout(ctx):
     srv.close()
     for v in vecs:
         v.close()
+
+
+def test_server_rns_requests_match_batch(ctx):
+    """RNS requests (centered lift, INTT_CENTER, both rescalings) submitted
+    to the server among transform requests give the words the same jobs
+    give in a batch."""
+    rng = np.random.default_rng(17)
+    n = 1 << 12
+    tabs = tables_for(12, 4)
+    srv = V.Server(ctx, streams=2, max_batch=8)
+    reqs, vecs = [], []
+    kinds = [(V.JOB_CENTER, [65537], 0), (V.JOB_INTT_CENTER, [2], 0),
+             (V.JOB_RESCALE, None, 1), (V.JOB_RESCALE_NTT, None, 1)]
+    for k in range(16):
+        op, prm, drop = kinds[k % 4]
+        a = ctx.vector(4 * n, False)
+        for i, t in enumerate(tabs):
+            a.fill_uniform(t.q, 7000 + 10 * k + i, i * n, n)
+        r1, r2 = ctx.vector((4 - drop) * n), ctx.vector((4 - drop) * n)
+        x, y = ctx.vector(4 * n, False), ctx.vector(4 * n)
+        for i, t in enumerate(tabs):
+            x.fill_uniform(t.q, 8000 + 10 * k + i, i * n, n)
+        rid = srv.submit_rns(op, a, r1, tabs, prm)
+        rid2 = srv.submit(V.JOB_NTT_FWD, x, None, y, tabs)
+        reqs.append((rid, rid2, op, prm, a, r1, r2))
+        vecs += [a, r1, r2, x, y]
+    for rid, rid2, op, prm, a, r1, r2 in reqs:
+        assert srv.wait(rid)[0] == VV.OK and srv.wait(rid2)[0] == VV.OK
+        b = V.Batch(ctx)
+        b.add_rns(op, a, r2, tabs, prm)
+        b.run()
+        b.close()
+        assert np.array_equal(r1.read(), r2.read()), op
+    with pytest.raises(V.VkhelError):
+        srv.submit_rns(V.JOB_RESCALE, reqs[0][4], reqs[0][4], tabs)
+    srv.close()
+    for v in vecs:
+        v.close()

@@ -1,5 +1,6 @@
 // Serving engine: continuous batching of NTT / INTT / polymul / pointwise
-// requests over device-resident operands.  The reference has no serving
+// and RNS (centered lift, INTT_CENTER, rescaling) requests over
+// device-resident operands.  The reference has no serving
 // layer (every vkhel_vector_* call blocks on a fence, src/vector.c:326).
 //
 // Requests are queued on submit and admitted in arrival order while the
@@ -227,15 +228,10 @@ void vkhel_server_destroy(struct vkhel_server *s) {
 	delete s;
 }
 
-uint64_t vkhel_server_submit(struct vkhel_server *s, int op,
-		const struct vkhel_vector *a, const struct vkhel_vector *b,
-		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,
-		uint32_t nlimbs) {
-	if (!s)
-		return fail(VKHEL_ERR_INVALID, "server_submit: null server"), 0;
-	Request r;
-	if (make_job(s->ctx, op, a, b, res, tables, nlimbs, &r.job) != VKHEL_OK)
-		return 0;
+}  // extern "C"
+
+// Queues a checked job; returns its id, or 0 with the error recorded.
+static uint64_t submit_request(vkhel_server *s, Request &&r) {
 	r.bytes = job_bytes(r.job);
 	r.info.submit_ns = now_ns();
 	uint64_t id;
@@ -261,6 +257,33 @@ uint64_t vkhel_server_submit(struct vkhel_server *s, int op,
 	}
 	s->cv_work.notify_one();
 	return id;
+}
+
+extern "C" {
+
+uint64_t vkhel_server_submit(struct vkhel_server *s, int op,
+		const struct vkhel_vector *a, const struct vkhel_vector *b,
+		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,
+		uint32_t nlimbs) {
+	if (!s)
+		return fail(VKHEL_ERR_INVALID, "server_submit: null server"), 0;
+	Request r;
+	if (make_job(s->ctx, op, a, b, res, tables, nlimbs, &r.job) != VKHEL_OK)
+		return 0;
+	return submit_request(s, std::move(r));
+}
+
+uint64_t vkhel_server_submit_rns(struct vkhel_server *s, int op,
+		const struct vkhel_vector *a, struct vkhel_vector *res,
+		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs,
+		const uint64_t *params) {
+	if (!s)
+		return fail(VKHEL_ERR_INVALID, "server_submit_rns: null server"), 0;
+	Request r;
+	if (make_rns_job(s->ctx, op, a, res, tables, nlimbs, params, &r.job) !=
+			VKHEL_OK)
+		return 0;
+	return submit_request(s, std::move(r));
 }
 
 int vkhel_server_wait(struct vkhel_server *s, uint64_t id, uint64_t timeout_ns,

@@ -77,6 +77,8 @@ SIGNATURES = {
     "vkhel_server_create": (_p, [_p, _p]),
     "vkhel_server_destroy": (None, [_p]),
     "vkhel_server_submit": (_u64, [_p, _int, _p, _p, _p, ctypes.POINTER(_p), _u32]),
+    "vkhel_server_submit_rns": (_u64, [_p, _int, _p, _p, ctypes.POINTER(_p), _u32,
+                                       _p]),
     "vkhel_server_wait": (_int, [_p, _u64, _u64, _p]),
     "vkhel_server_cancel": (_int, [_p, _u64]),
     "vkhel_server_drain": (_int, [_p]),

@@ -427,6 +427,19 @@ class Server:
             raise _lib.VkhelError(ERR_INVALID, "submit failed")
         return int(rid)
 
+    def submit_rns(self, op, a, res, tables, params=None):
+        """vkhel_server_submit_rns: the RNS jobs of Batch.add_rns."""
+        tabs = list(tables)
+        arr = (ctypes.c_void_p * len(tabs))(*[t.handle for t in tabs])
+        prm = None if params is None else np.ascontiguousarray(params, dtype=np.uint64)
+        rid = _L().vkhel_server_submit_rns(self._h, op, a.handle, res.handle, arr,
+                                           len(tabs),
+                                           None if prm is None else _u64p(prm))
+        if not rid:
+            _lib.check()
+            raise _lib.VkhelError(ERR_INVALID, "submit failed")
+        return int(rid)
+
     def wait(self, rid, timeout_ns=0):
         """(status, info); status ERR_TIMEOUT leaves the request pending."""
         info = RequestInfo()
