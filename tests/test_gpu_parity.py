@@ -1074,3 +1074,37 @@ def test_batch_mixes_every_job_kind(ctx):
     batch.close()
     for x in keep:
         x.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_rns_rescale_ntt_pass_modes(ctx, mode):
+    """The rescaling prologue and epilogue in both two-pass kernel forms
+    (one workgroup per polynomial running both passes, or split passes)
+    give the coefficient-form job's words, transformed."""
+    logn = 14
+    n = 1 << logn
+    qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in (61, 50, 59)]
+    ws = [O.root_2n(q, n) for q in qs]
+    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+    spec = np.concatenate([O.fill_uniform(n, q, 77 + i) for i, q in enumerate(qs)])
+    coeffs = np.concatenate([O.ntt_inverse(spec[i * n:(i + 1) * n], q, w)
+                             for i, (q, w) in enumerate(zip(qs, ws))])
+    a, c = vec(ctx, spec), vec(ctx, coeffs)
+    r, rc = ctx.vector(2 * n), ctx.vector(2 * n)
+    ctx.set_pass_mode(logn, mode)
+    try:
+        b = V.Batch(ctx)
+        b.add_rns(V.JOB_RESCALE_NTT, a, r, tabs)
+        b.add_rns(V.JOB_RESCALE, c, rc, tabs)
+        b.run()
+        got, flat = r.read(), rc.read()
+        for i in range(2):
+            assert np.array_equal(got[i * n:(i + 1) * n],
+                                  O.ntt_forward(flat[i * n:(i + 1) * n], qs[i], ws[i])), i
+        b.close()
+    finally:
+        ctx.set_pass_mode(logn, 0)
+        for v in (a, c, r, rc):
+            v.close()
+        for t in tabs:
+            t.close()
