@@ -164,7 +164,8 @@ enum vkhel_job_op {
 	                                             params[3i+2])             */
 	/* fused: res_i = elemmod(INTT(a_i), params[0], q_i), the centered lift
 	 * applied as the inverse transform stores its results (one pass over
-	 * memory less than INTT then CENTER); n from 2^3 to 2^20 */
+	 * memory less than INTT then CENTER) for n from 2^3 to 2^20; other
+	 * sizes run the lift as a pointwise pass after the transform */
 	VKHEL_JOB_INTT_CENTER = 8,
 	/* RNS rescaling by the last modulus (nlimbs >= 2, params unused):
 	 * res_i = (a_i - [a_last]) * q_last^-1 mod q_i for i < nlimbs - 1, where

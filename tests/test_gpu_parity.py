@@ -816,18 +816,32 @@ def test_fused_intt_center_matches_intt_then_elemmod(ctx, p, logn, tb, mode):
             t.close()
 
 
-def test_fused_intt_center_rejects_unsupported_sizes(ctx):
-    for n in (2, 4):
-        q = O.primes_ntt(1 << 50, 2 * n, 1)[0]
-        t = V.NttTables(n, q, O.root_2n(q, n))
-        a, r = ctx.vector(n), ctx.vector(n)
+@pytest.mark.parametrize("p", [2, 3, 65537])
+def test_intt_center_outside_fused_sizes(ctx, p):
+    """n = 2, 4 and 2^21, outside the fused kernels: the inverse transform,
+    then the centered lift as a pointwise pass, in the same job."""
+    for logn in (1, 2, 21):
+        n = 1 << logn
+        qs = [O.primes_ntt(1 << b, 2 * n, 1)[0] for b in (50, 61)]
+        ws = [O.root_2n(q, n) for q in qs]
+        tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+        x = np.concatenate([O.fill_uniform(n, q, 40 + i) for i, q in enumerate(qs)])
+        a, r = vec(ctx, x), ctx.vector(2 * n)
         b = V.Batch(ctx)
-        with pytest.raises(V.VkhelError):
-            b.add_rns(V.JOB_INTT_CENTER, a, r, [t], [3])  # n below 2^3
+        b.add_rns(V.JOB_INTT_CENTER, a, r, tabs, [p])
+        b.run()
+        got = r.read()
+        for i, (q, w) in enumerate(zip(qs, ws)):
+            sl = slice(i * n, (i + 1) * n)
+            assert np.array_equal(got[sl], O.elemmod(O.ntt_inverse(x[sl], q, w), p, q)), (n, i)
         b.close()
-        a.close()
-        r.close()
-        t.close()
+        for v in (a, r):
+            v.close()
+        for t in tabs:
+            t.close()
+
+
+def test_intt_center_rejects_bad_modulus(ctx):
     n = 1 << 8
     q = O.primes_ntt(1 << 50, 2 * n, 1)[0]
     t = V.NttTables(n, q, O.root_2n(q, n))

@@ -175,10 +175,6 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		(op == VKHEL_JOB_GTADD ? 2 : 3) * (size_t) nlimbs;
 	if (center && params[0] < 2)
 		return fail(VKHEL_ERR_INVALID, "rns job: center needs p >= 2");
-	if (op == VKHEL_JOB_INTT_CENTER &&
-			!ntt_fused_supported((uint32_t) (bitlen(tables[0]->n) - 1)))
-		return fail(VKHEL_ERR_INVALID,
-				"rns job: intt_center needs n from 2^3 to 2^20");
 	if (op == VKHEL_JOB_GTSUB)
 		for (uint32_t i = 0; i < nlimbs; i++)
 			if (params[3 * i + 2] < 2)
@@ -235,6 +231,21 @@ static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
 		segs.back().count++;
 	}
 	return segs;
+}
+
+// elemmod(a, pm, q) over n words into out (src/vector.c:342-386: the
+// reference's dispatch, modulo-two kernel for pm = 2).
+static PointTask center_task(const uint64_t *a, uint64_t *out, uint64_t n,
+		uint64_t q, uint64_t pm) {
+	PointTask p{a, nullptr, out, n, 0, 0, q / 2, 0, 0, kPtModTwo};
+	if (pm != 2) {
+		p.op = kPtGtSub;
+		p.diff = q;
+		p.q = pm;
+		p.mu = barrett_mu(pm);
+		p.shift = (uint32_t) (bitlen(pm) - 2);
+	}
+	return p;
 }
 
 int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
@@ -322,9 +333,18 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				break;
 			}
 			case VKHEL_JOB_INTT_CENTER: {
-				// elemmod applied by the inverse kernel as it stores
+				// elemmod applied by the inverse kernel as it stores; sizes
+				// outside the fused kernels run it as a pointwise pass after
+				// the inverse transforms
 				NttTask inv = make_task(*t, j.a->dev + off, j.res->dev + off, true);
 				const uint64_t pm = j.params[0];
+				if (!ntt_fused_supported(t->c.logn)) {
+					ts.inv.push_back(inv);
+					ts.post.push_back(center_task(j.res->dev + off, j.res->dev + off, n,
+							q, pm));
+					ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
+					break;
+				}
 				inv.c.post_p = pm;
 				if (pm != 2) {
 					inv.c.post_mu = barrett_mu(pm);
@@ -381,23 +401,16 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				break;
 			}
 			case VKHEL_JOB_CENTER:
+				ts.pt.push_back(center_task(j.a->dev + off, j.res->dev + off, n, q,
+						j.params[0]));
+				break;
 			case VKHEL_JOB_GTADD:
 			case VKHEL_JOB_GTSUB: {
-				// src/vector.c:342-511: the same per-limb push constants
+				// src/vector.c:429-511: the same per-limb push constants
 				PointTask p{j.a->dev + off, nullptr, j.res->dev + off, n, 0, 0, 0, 0,
 					0, kPtGtAdd};
 				const uint64_t *pr = j.params.data();
-				if (j.op == VKHEL_JOB_CENTER) {
-					const uint64_t pm = pr[0];
-					p.bound = q / 2;
-					if (pm == 2) {
-						p.op = kPtModTwo;
-					} else {
-						p.op = kPtGtSub;
-						p.diff = q;
-						p.q = pm;
-					}
-				} else if (j.op == VKHEL_JOB_GTADD) {
+				if (j.op == VKHEL_JOB_GTADD) {
 					p.bound = pr[2 * l];
 					p.diff = pr[2 * l + 1];
 				} else {
