@@ -522,6 +522,37 @@ def rescale_ntt_leg(V, ctx, ev, logn=16, limbs=8, polys=64, reps=5):
     return res
 
 
+def pointwise_leg(V, ctx, ev, words=1 << 27, reps=5):
+    """The reference's pointwise ops as single calls on 1 GiB vectors (one
+    launch each, synchronous): algorithmic bytes (24 per element for the
+    two-input ops, 16 for the one-input ones) over the HIP-event time of
+    the call, against the 8 TB/s peak."""
+    q = primes_ntt(1 << 60, 1 << 17, 1)[0]
+    a, b, r = (ctx.vector(words, False) for _ in range(3))
+    a.fill_uniform(q, 1)
+    b.fill_uniform(q, 2)
+    ops = {"elemmul": (24, lambda: a.elemmul(b, r, q)),
+           "elemfma_m1": (24, lambda: a.elemfma(b, r, 1, q)),
+           "elemmod_p65537": (16, lambda: a.elemmod(r, 65537, q)),
+           "elemgtadd": (16, lambda: a.elemgtadd(r, q // 2, 12345)),
+           "elemgtsub": (16, lambda: a.elemgtsub(r, q // 2, 12345, 65537))}
+    out = {"workload": f"single calls on {words} words (q < 2^60)"}
+    for name, (bpe, fn) in ops.items():
+        fn()
+        e0, e1 = ev.new(), ev.new()
+        ev.record(e0, ctx.stream)
+        for _ in range(reps):
+            fn()
+        ev.record(e1, ctx.stream)
+        ms = ev.ms(e0, e1) / reps
+        gbs = bpe * words / ms / 1e6
+        out[name] = {"ms": round(ms, 4), "alg_GBps": round(gbs, 1),
+                     "frac_of_peak": round(gbs / HBM_PEAK_GBS, 3)}
+    for v in (a, b, r):
+        v.close()
+    return out
+
+
 def pcie_inclusive(V, ctx, logn=16, primes=8, per_prime=64, chunks=8,
                    per_call_sample=32):
     """Host-resident inputs and outputs (the headline batch, n = 2^16,
@@ -792,6 +823,7 @@ def main():
         extra["polymul"] = polymul_leg(V, ctx, ev)
         extra["intt_center"] = intt_center_leg(V, ctx, ev)
         extra["rescale_ntt"] = rescale_ntt_leg(V, ctx, ev)
+        extra["pointwise"] = pointwise_leg(V, ctx, ev)
         extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
         extra["plain_polymul_n4096"] = plain_polymul_n4096(V, ctx)
         extra["pcie_inclusive"] = pcie_inclusive(V, ctx)
