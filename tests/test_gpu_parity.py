@@ -1122,3 +1122,42 @@ def test_rns_rescale_ntt_pass_modes(ctx, mode):
             v.close()
         for t in tabs:
             t.close()
+
+
+def test_batch_with_70000_limbs(ctx):
+    """More limbs than a grid dimension holds (65535): a MUL job and an
+    NTT_FWD job of 70000 limbs at n = 8 (one pointwise row and one
+    transform workgroup per limb), and an NTT_FWD job of 70000 limbs at
+    n = 2^12 on 2^11 tiles (two passes, run in chunks), sampled against the
+    oracle."""
+    L = 70000
+    q = O.primes_ntt(1 << 50, 1 << 13, 1)[0]
+    rng = np.random.default_rng(70000)
+    sample = rng.choice(L, 24, replace=False).tolist() + [0, 65534, 65535, L - 1]
+    for n in (8, 1 << 12):
+        t = V.NttTables(n, q, O.root_2n(q, n))
+        x = rng.integers(0, q, L * n, dtype=np.uint64)
+        y = rng.integers(0, q, L * n, dtype=np.uint64) if n == 8 else None
+        a, r = vec(ctx, x), ctx.vector(L * n)
+        ctx.set_tile_bits(12, 11)
+        try:
+            b = V.Batch(ctx)
+            b.add(V.JOB_NTT_FWD, a, None, r, [t] * L)
+            if n == 8:
+                bv, m = vec(ctx, y), ctx.vector(L * n)
+                b.add(V.JOB_MUL, a, bv, m, [t] * L)
+            b.run()
+            got = r.read()
+            for i in sample:
+                sl = slice(i * n, (i + 1) * n)
+                assert np.array_equal(got[sl], O.ntt_forward(x[sl], q, t.w)), (n, i)
+            if n == 8:
+                assert np.array_equal(m.read(), O.elemmul(x, y, q))
+                bv.close()
+                m.close()
+            b.close()
+        finally:
+            ctx.set_tile_bits(12, 0)
+        a.close()
+        r.close()
+        t.close()
