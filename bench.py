@@ -681,7 +681,35 @@ def cpu_baseline(logn, qs, ws, target_cpu_s=16.0, min_runs=5):
                       f"C (oracle/vkhel_oracle.c, -O3 -march=x86-64-v3) over "
                       f"{threads} threads, {sum(runs):.2f} s wall",
             "one_core_gcoeff_s": round(2 * 4 * n / t1 / 1e9, 5),
-            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "polymul": cpu_polymul_baseline(threads)}
+
+
+def cpu_polymul_baseline(threads, logn=15, primes=16, target_cpu_s=8.0):
+    """The polymul workload's CPU figure (n = 2^15, 16 primes < 2^55): the
+    oracle's scalar forward, forward, pointwise product and inverse per
+    pair on `threads` host threads over a bounded sample of pairs, output
+    coefficients per second (median of 3 passes)."""
+    from oracle import oracle as O
+    n = 1 << logn
+    qs = primes_ntt(1 << 55, 2 * n, primes)
+    ws = [root_2n(q, n) for q in qs]
+    probe_a = np.stack([uniform_mod(qs[i], n, 300 + i) for i in range(2)])
+    probe_b = np.stack([uniform_mod(qs[i], n, 400 + i) for i in range(2)])
+    t1, _ = O.bench_polymul_batch(probe_a, probe_b, n, qs[:2], ws[:2], 1)
+    per_pair = max(t1 / 2, 1e-6)
+    pairs = int(min(max(threads, target_cpu_s / 3 / per_pair), 1024))
+    pairs = max(primes, pairs - pairs % primes)
+    a = np.stack([uniform_mod(qs[i % primes], n, 500 + i) for i in range(pairs)])
+    b = np.stack([uniform_mod(qs[i % primes], n, 9000 + i) for i in range(pairs)])
+    runs = sorted(O.bench_polymul_batch(a, b, n, qs, ws, threads)[0] for _ in range(3))
+    med = runs[1]
+    return {"value": round(pairs * n / med / 1e9, 4), "unit": "Gcoeff/s (output)",
+            "cores": threads, "kind": "port",
+            "sample": f"median of 3 passes over {pairs} pairs of n=2^{logn} "
+                      f"({primes} primes <2^55): forward, forward, product, inverse "
+                      f"per pair, scalar C over {threads} threads",
+            "one_core_value": round(2 * n / t1 / 1e9, 5)}
 
 
 def plain_polymul_n4096(V, ctx):

@@ -55,6 +55,11 @@ def lib():
             "or_bench_ntt_batch": (ctypes.c_double, [_p64, ctypes.c_size_t, _u64,
                                                      _p64, _p64, ctypes.c_size_t,
                                                      ctypes.c_int]),
+            "or_bench_polymul_batch": (ctypes.c_double, [_p64, _p64, _p64,
+                                                         ctypes.c_size_t, _u64,
+                                                         _p64, _p64,
+                                                         ctypes.c_size_t,
+                                                         ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -218,3 +223,15 @@ def bench_ntt_batch(polys, n, qs, ws, threads):
     qs, ws = _arr(qs), _arr(ws)
     return float(lib().or_bench_ntt_batch(_ptr(polys), polys.size // n, n,
                                           _ptr(qs), _ptr(ws), qs.size, threads))
+
+
+def bench_polymul_batch(a, b, n, qs, ws, threads):
+    """Products of the rows of ``a`` and ``b`` (pair p modulo qs[p % L]);
+    (seconds, products)."""
+    assert a.dtype == np.uint64 and a.flags.c_contiguous and a.shape == b.shape
+    out = np.empty_like(a)
+    qs, ws = _arr(qs), _arr(ws)
+    secs = float(lib().or_bench_polymul_batch(_ptr(a), _ptr(np.ascontiguousarray(b)),
+                                              _ptr(out), a.size // n, n, _ptr(qs),
+                                              _ptr(ws), qs.size, threads))
+    return secs, out

@@ -653,3 +653,61 @@ double or_bench_ntt_batch(uint64_t *polys, size_t npolys, uint64_t n,
 	free(args);
 	return (double) (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
 }
+
+struct polymul_arg {
+	const uint64_t *a, *b;
+	uint64_t *out;
+	size_t first, last;
+	uint64_t n;
+	const uint64_t *qs;
+	size_t nprimes;
+	struct tables *tabs;
+};
+
+static void *polymul_worker(void *p) {
+	struct polymul_arg *g = p;
+	uint64_t *fb = malloc(g->n * sizeof(uint64_t));
+	for (size_t k = g->first; k < g->last; k++) {
+		const size_t pi = k % g->nprimes;
+		const uint64_t q = g->qs[pi];
+		const struct tables *t = &g->tabs[pi];
+		uint64_t *fa = g->out + k * g->n;
+		memcpy(fa, g->a + k * g->n, g->n * sizeof(uint64_t));
+		memcpy(fb, g->b + k * g->n, g->n * sizeof(uint64_t));
+		fwd_fast(fa, g->n, q, t->r, t->rb);
+		fwd_fast(fb, g->n, q, t->r, t->rb);
+		or_elemmul(fa, fb, fa, g->n, q);
+		const uint64_t ninv = or_invmod(g->n, q);
+		inv_fast(fa, g->n, q, t->ir, t->irb, ninv, or_barrett_factor(ninv, q));
+	}
+	free(fb);
+	return NULL;
+}
+
+double or_bench_polymul_batch(const uint64_t *a, const uint64_t *b,
+		uint64_t *out, size_t npairs, uint64_t n, const uint64_t *qs,
+		const uint64_t *ws, size_t nprimes, int threads) {
+	if (!pow2_ok(n) || nprimes == 0 || threads < 1)
+		return -1.0;
+	struct tables *tabs = calloc(nprimes, sizeof(*tabs));
+	for (size_t i = 0; i < nprimes; i++)
+		tables_alloc(&tabs[i], n, qs[i], ws[i]);
+	pthread_t *th = calloc((size_t) threads, sizeof(pthread_t));
+	struct polymul_arg *args = calloc((size_t) threads, sizeof(*args));
+	struct timespec t0, t1;
+	clock_gettime(CLOCK_MONOTONIC, &t0);
+	for (int i = 0; i < threads; i++) {
+		args[i] = (struct polymul_arg) {a, b, out, npairs * (size_t) i / threads,
+			npairs * (size_t) (i + 1) / threads, n, qs, nprimes, tabs};
+		pthread_create(&th[i], NULL, polymul_worker, &args[i]);
+	}
+	for (int i = 0; i < threads; i++)
+		pthread_join(th[i], NULL);
+	clock_gettime(CLOCK_MONOTONIC, &t1);
+	for (size_t i = 0; i < nprimes; i++)
+		free(tabs[i].r);
+	free(tabs);
+	free(th);
+	free(args);
+	return (double) (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+}

@@ -122,6 +122,13 @@ void or_fill_uniform(uint64_t *out, size_t len, uint64_t q, uint64_t seed);
 double or_bench_ntt_batch(uint64_t *polys, size_t npolys, uint64_t n,
 		const uint64_t *qs, const uint64_t *ws, size_t nprimes,
 		int threads);
+/* Negacyclic products out_p = a_p * b_p mod (X^n + 1, q) of `npairs` pairs
+ * of n words (forward transforms of both, pointwise product, inverse),
+ * pair p using prime index p % nprimes; `threads` POSIX threads.  Returns
+ * wall seconds. */
+double or_bench_polymul_batch(const uint64_t *a, const uint64_t *b,
+		uint64_t *out, size_t npairs, uint64_t n, const uint64_t *qs,
+		const uint64_t *ws, size_t nprimes, int threads);
 
 #ifdef __cplusplus
 }

@@ -283,3 +283,17 @@ def test_cpu_bench_runner_matches_oracle():
     assert secs > 0
     # forward+inverse is the identity
     assert np.array_equal(polys, before)
+
+
+def test_cpu_polymul_runner_matches_schoolbook():
+    """The threaded polymul timing runner's products against the schoolbook
+    negacyclic product (pair p modulo qs[p % 3])."""
+    n = 64
+    qs = O.primes_ntt(1 << 55, 2 * n, 3)
+    ws = [O.root_2n(q, n) for q in qs]
+    a = np.stack([O.fill_uniform(n, qs[i % 3], 10 + i) for i in range(7)])
+    b = np.stack([O.fill_uniform(n, qs[i % 3], 50 + i) for i in range(7)])
+    secs, out = O.bench_polymul_batch(a, b, n, qs, ws, threads=3)
+    assert secs > 0
+    for i in range(7):
+        assert np.array_equal(out[i], O.negacyclic_schoolbook(a[i], b[i], qs[i % 3])), i
