@@ -20,6 +20,9 @@ Also reported, each with its own timing:
     pointwise, INTT) as one batch, vs per call;
   * intt_center: INTT + centered lift into Z_p of the headline batch,
     fused in the inverse kernel vs as two batches;
+  * rescale_ntt: RNS rescaling of NTT-form polynomials, one job vs the
+    three-batch composition;
+  * pointwise: the reference's pointwise ops as single calls on 1 GiB;
   * uneven_batch: 1000 seeded jobs, n = 2^10..2^17, 1..24 primes < 2^60,
     ops NTT / INTT / polymul / add / mul, as one batch and per call;
   * serving: the same 1000 jobs through the serving engine at Poisson
@@ -28,7 +31,9 @@ Also reported, each with its own timing:
     memory (per-call reference API, and a pipelined batch with transfers
     on copy streams) -- reported beside `value`, never as it;
   * cpu_baseline: the oracle's scalar C NTT on the host cores (bounded
-    sample).
+    sample), the polymul workload's CPU figure, and one n = 4096 polymul
+    on one CPU thread against the per-call GPU path (the only leg that
+    runs the oracle).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -654,11 +659,14 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(logn, qs, ws, target_cpu_s=16.0, min_runs=5):
+def cpu_baseline(logn, qs, ws, V=None, ctx=None, target_cpu_s=16.0,
+                 min_runs=5):
     """Oracle scalar NTT (fwd+inv) on host cores over a bounded sample: up
     to 4096 resident polynomials, at least `min_runs` timed passes after a
     warm-up pass (median reported) and about target_cpu_s thread-seconds
-    of work."""
+    of work.  Also the polymul workload's CPU figure and, given a context,
+    the single n = 4096 polymul on one CPU thread against the per-call GPU
+    path.  The only leg that runs the oracle."""
     from oracle import oracle as O
     n = 1 << logn
     threads = max(1, min(os.cpu_count() or 1, 16))
@@ -682,7 +690,9 @@ def cpu_baseline(logn, qs, ws, target_cpu_s=16.0, min_runs=5):
                       f"{threads} threads, {sum(runs):.2f} s wall",
             "one_core_gcoeff_s": round(2 * 4 * n / t1 / 1e9, 5),
             "nproc": os.cpu_count(), "cpu_model": cpu_model(),
-            "polymul": cpu_polymul_baseline(threads)}
+            "polymul": cpu_polymul_baseline(threads),
+            "plain_polymul_n4096": (plain_polymul_n4096(V, ctx)
+                                    if ctx is not None else None)}
 
 
 def cpu_polymul_baseline(threads, logn=15, primes=16, target_cpu_s=8.0):
@@ -853,10 +863,9 @@ def main():
         extra["rescale_ntt"] = rescale_ntt_leg(V, ctx, ev)
         extra["pointwise"] = pointwise_leg(V, ctx, ev)
         extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
-        extra["plain_polymul_n4096"] = plain_polymul_n4096(V, ctx)
         extra["pcie_inclusive"] = pcie_inclusive(V, ctx)
 
-    cpu = None if args.no_cpu else cpu_baseline(args.logn, qs, ws)
+    cpu = None if args.no_cpu else cpu_baseline(args.logn, qs, ws, V, ctx)
     tb = ctx.tile_bits(args.logn)
     traffic, prof = traffic_from_profile(args.logn, npoly, tb)
     out = {
