@@ -35,6 +35,9 @@
  *   or_ntt_inverse on inputs >= q: parity unpinned (the reference loop at
  *       nttrevbutterfly.comp:44-48 is undefined-ish there); the oracle
  *       rejects such inputs.
+ *   or_bench_ntt_batch, or_bench_polymul_batch (timing runners for the
+ *       CPU baseline): checked by the forward+inverse identity and against
+ *       schoolbook products.
  */
 #ifndef VKHEL_ORACLE_H
 #define VKHEL_ORACLE_H
