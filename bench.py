@@ -799,6 +799,13 @@ def main():
     ap.add_argument("--polymul-only", action="store_true",
                     help="only run the polymul leg (after tuning) and print "
                          "its JSON object")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (library default tiles), e.g. so "
+                         "that a trace run and a counter run of a profile use "
+                         "the same kernels")
+    ap.add_argument("--uneven-only", action="store_true",
+                    help="only run the uneven-batch leg (1000 mixed jobs as "
+                         "one batch, after tuning) and print its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 copies of 256 MiB (elemgtadd with bound "
                          "2^64-1) to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -822,12 +829,25 @@ def main():
                           "launches": 3}))
         return
     t_tune = time.perf_counter()
-    ctx.autotune(12, 17)
+    if not args.no_tune:
+        ctx.autotune(12, 17)
     if args.tile_bits:
         ctx.set_tile_bits(args.logn, args.tile_bits)
         ctx.set_pass_mode(args.logn, 2)
     if args.polymul_only:
         print(json.dumps({"polymul": polymul_leg(V, ctx, ev)}))
+        return
+    if args.uneven_only:
+        jobs, _ = make_uneven_jobs(V, ctx, 1000)
+        coeffs = sum(job_coeffs(V, j) for j in jobs)
+        bt = V.Batch(ctx)
+        for j in jobs:
+            bt.add(*j)
+        ms = timed_launches(ctx, bt, 3, ev)
+        print(json.dumps({"uneven_batch": {"batch_ms": round(ms, 3),
+                                           "transform_coeffs": coeffs,
+                                           "batched_gcoeff_s": round(coeffs / ms / 1e6, 3)}}))
+        bt.close()
         return
     tuned = {l: ctx.tile_bits(l) for l in range(12, 18)}
     modes = {l: ctx.pass_mode(l) for l in range(12, 18)}
