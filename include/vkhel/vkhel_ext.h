@@ -303,6 +303,26 @@ int vkhel_server_replay(struct vkhel_server *, const struct vkhel_job_desc *jobs
 		uint64_t njobs, double rate_per_s, uint64_t seed,
 		struct vkhel_request_info *infos);
 
+/* ---- polynomial products without device allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Computes what a one-job batch with VKHEL_JOB_POLYMUL
+ * computes (res = a*b mod (X^n + 1, q_i) per limb, the same kernels and
+ * bit-identical words), enqueued on `stream` (a hipStream_t; null: the
+ * context's stream) without waiting.  It allocates nothing: b's spectra go
+ * to a static device array of VKHEL_POLYMUL_NOALLOC_MAX_WORDS words per GPU
+ * (32 MiB, held until the process ends) and every kernel takes its task by
+ * value.  So nlimbs * n must not exceed that maximum, and every limb's
+ * tables must already be resident on the context (vkhel_ext_tables_upload,
+ * or any earlier transform with them); otherwise VKHEL_ERR_INVALID and
+ * nothing is enqueued.  Every call on one GPU shares its array: calls that
+ * overlap on one GPU (other streams, other threads) race, so serialise them
+ * on one stream.  res must not alias b. */
+#define VKHEL_POLYMUL_NOALLOC_MAX_WORDS (1ull << 22)
+int vkhel_ext_polymul_noalloc(struct vkhel_ctx *, const struct vkhel_vector *a,
+		const struct vkhel_vector *b, struct vkhel_vector *res,
+		struct vkhel_ntt_tables *const *tables, uint32_t nlimbs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -196,11 +196,23 @@ int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn);
 // from out in the fused kernel; one-tile sizes take col_tasks null.
 int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
 		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st);
+// One fused polynomial product without task arrays: the column task (two-
+// pass sizes, cols true) and the product task passed by value.
+int launch_polymul_one(const NttTask &col, const NttTask &pm, bool cols,
+		uint32_t logn, int tile_bits, int mc, hipStream_t st);
+// One fused transform of task k (its in2 / epilogue fields honoured), by
+// value: reads no task array.
+int launch_task_fused(const vkhel_ctx *ctx, const NttTask &k, bool inverse,
+		hipStream_t st);
 // Tile size (log2 words) used for log2 n on this context.
 int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn);
 bool ntt_fused_supported(uint32_t logn);
 // Radix-2 path for one task (sizes the fused kernels do not cover).
 int launch_task_ntt(const NttTask &k, bool inverse, hipStream_t s);
+
+// Looks up this device's static scratch array (noalloc.hip) with the
+// device current; called once per context creation.
+int noalloc_init(int device);
 
 // Pointwise launchers (elementwise.hip).
 int launch_elemmul(const uint64_t *a, const uint64_t *b, uint64_t *out,

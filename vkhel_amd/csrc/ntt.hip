@@ -258,18 +258,24 @@ int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn) {
 	return fused_entry(logn, tb).pm_ok ? tb : 0;
 }
 
-int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
+// Fused polynomial products from device task arrays (col_tasks null for
+// one-tile sizes) or, with both arrays null, from `ucol` / `upm` for one
+// product (the kernels' uniform mode, nothing read from device memory).
+static int polymul_impl(const NttTask &ucol, const NttTask &upm,
+		const NttTask *col_tasks, const NttTask *pm_tasks, bool cols,
 		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st) {
 	if (ntasks == 0)
 		return VKHEL_OK;
+	if (!fused_supported(logn))
+		return fail(VKHEL_ERR_INVALID, "fused polymul: unsupported log2 n = %u",
+				logn);
 	const FusedEntry &e = fused_entry(logn, tile_bits);
-	if (!fused_supported(logn) || !e.pm_ok || ((e.tiles > 1) != (col_tasks != nullptr)))
+	if (!e.pm_ok || (e.tiles > 1) != cols)
 		return fail(VKHEL_ERR_INVALID,
 				"fused polymul: unsupported log2 n = %u on 2^%d tiles", logn,
 				tile_bits);
-	const NttTask none{};
 	if (e.tiles == 1) {
-		hipLaunchKernelGGL(e.pm[mc], dim3(ntasks), dim3(e.threads), 0, st, none,
+		hipLaunchKernelGGL(e.pm[mc], dim3(ntasks), dim3(e.threads), 0, st, upm,
 				pm_tasks, 0u);
 	} else {
 		// chunked as the split transforms (kChunkWords): the row kernel
@@ -278,16 +284,39 @@ int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
 				std::min<uint64_t>(kChunkWords >> logn, 65535));
 		for (uint32_t first = 0; first < ntasks; first += chunk) {
 			const dim3 grid(e.tiles, std::min(chunk, ntasks - first));
-			hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, none,
-					col_tasks + first, 0u);
-			hipLaunchKernelGGL(e.pm[mc], grid, dim3(e.threads), 0, st, none,
-					pm_tasks + first, 0u);
-			hipLaunchKernelGGL(e.inv_a[mc], grid, dim3(e.threads), 0, st, none,
-					pm_tasks + first, 1u);
+			const NttTask *ct = col_tasks ? col_tasks + first : nullptr;
+			const NttTask *pt = pm_tasks ? pm_tasks + first : nullptr;
+			hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, ucol, ct,
+					0u);
+			hipLaunchKernelGGL(e.pm[mc], grid, dim3(e.threads), 0, st, upm, pt, 0u);
+			hipLaunchKernelGGL(e.inv_a[mc], grid, dim3(e.threads), 0, st, upm, pt,
+					1u);
 		}
 	}
 	return hip_ok(hipGetLastError(), "fused polymul launch") ? VKHEL_OK
 		: VKHEL_ERR_DEVICE;
+}
+
+int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
+		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st) {
+	const NttTask none{};
+	return polymul_impl(none, none, col_tasks, pm_tasks, col_tasks != nullptr,
+			ntasks, logn, tile_bits, mc, st);
+}
+
+int launch_polymul_one(const NttTask &col, const NttTask &pm, bool cols,
+		uint32_t logn, int tile_bits, int mc, hipStream_t st) {
+	return polymul_impl(col, pm, nullptr, nullptr, cols, 1, logn, tile_bits, mc,
+			st);
+}
+
+int launch_task_fused(const vkhel_ctx *ctx, const NttTask &k, bool inverse,
+		hipStream_t st) {
+	const uint32_t logn = k.c.logn;
+	if (!fused_supported(logn))
+		return fail(VKHEL_ERR_INVALID, "fused NTT: unsupported log2 n = %u", logn);
+	return launch_fused_impl(k, nullptr, 1, logn, tile_bits_for(ctx, logn),
+			use_split(ctx, logn), inverse, modulus_class(k.c.q), st);
 }
 
 int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,

@@ -149,6 +149,12 @@ struct vkhel_ctx *vkhel_ctx_create() {
 	}
 	ctx->device_name = p.gcnArchName;
 	ctx->cu_count = p.multiProcessorCount;
+	if (noalloc_init(ctx->device) != VKHEL_OK) {
+		(void) hipStreamDestroy(ctx->stream);
+		(void) hipStreamDestroy(ctx->table_stream);
+		delete ctx;
+		return nullptr;
+	}
 	std::lock_guard<std::mutex> g(g_ctx_mu);
 	g_ctxs.insert(ctx);
 	return ctx;

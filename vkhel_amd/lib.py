@@ -96,6 +96,8 @@ SIGNATURES = {
     "vkhel_batch_query": (_int, [_p]),
     "vkhel_batch_run": (_int, [_p]),
     "vkhel_batch_num_jobs": (_u64, [_p]),
+    "vkhel_ext_polymul_noalloc": (_int, [_p, _p, _p, _p, ctypes.POINTER(_p), _u32,
+                                         _p]),
 }
 
 

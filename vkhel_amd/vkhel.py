@@ -85,6 +85,18 @@ class Context:
     def vector(self, length, zero=True):
         return Vector(self, length, zero)
 
+    def polymul_noalloc(self, a, b, res, tables, stream=None):
+        """vkhel_ext_polymul_noalloc: res = a*b mod (X^n + 1, q_i) per limb,
+        b's spectra in the library's static per-GPU scratch (no device
+        allocation); enqueued on `stream` (a hipstream.Stream; None: the
+        context's stream) without waiting.  Tables must be resident
+        (NttTables.upload)."""
+        tabs = list(tables)
+        arr = (ctypes.c_void_p * len(tabs))(*[t.handle for t in tabs])
+        s = stream.handle if stream is not None else None
+        _lib.check(_L().vkhel_ext_polymul_noalloc(self._h, a.handle, b.handle,
+                                                  res.handle, arr, len(tabs), s))
+
     def from_array(self, arr):
         a = np.ascontiguousarray(arr, dtype=np.uint64)
         v = Vector(self, a.size, zero=False)
@@ -144,6 +156,10 @@ class NttTables:
         _lib.check(_L().vkhel_ext_tables_read_device(
             ctx.handle, self._h, *[_u64p(a) for a in out]))
         return tuple(out)
+
+    def upload(self, ctx):
+        """vkhel_ext_tables_upload: make the device copy on ctx now."""
+        _lib.check(_L().vkhel_ext_tables_upload(ctx.handle, self._h))
 
     def arrays(self):
         """(roots, inv_roots, roots_bf, inv_roots_bf), reference layout."""
