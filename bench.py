@@ -4,8 +4,10 @@ Headline workload (`value`): 8 RNS primes q_i < 2^50 with q_i = 1 mod 2n,
 64 polynomials per prime (512 polynomials), n = 2^16 uint64 coefficients
 uniform in [0, q_i).  One step = forward NTT of all 512 polynomials then
 inverse NTT of all 512 (in place, so every step sees the same data), issued
-as one replayed hipGraph on the library's stream.  `value` is transform-
-coefficients per second (2 * 512 * n per step) with inputs resident in HBM.
+as one batch launch on the library's stream (its few kernel launches are
+enqueued directly; batches of many launches replay a captured hipGraph).
+`value` is transform-coefficients per second (2 * 512 * n per step) with
+inputs resident in HBM.
 
 Also reported, each with its own timing:
   * roofline of the dominant kernel, the forward NTT of the batch at
@@ -26,7 +28,9 @@ Also reported, each with its own timing:
   * uneven_batch: 1000 seeded jobs, n = 2^10..2^17, 1..24 primes < 2^60,
     ops NTT / INTT / polymul / add / mul, as one batch and per call;
   * serving: the same 1000 jobs through the serving engine at Poisson
-    arrival rates, p50/p99 latency beside a FIFO of plain per-call service;
+    arrival rates from 10% to 150% of the batched job rate (three runs per
+    point), p50/p99 latency beside a plain per-call server at the same
+    rates; every run checked against the batch's words;
   * pcie_inclusive: the headline batch with inputs and outputs in host
     memory (per-call reference API, and a pipelined batch with transfers
     on copy streams) -- reported beside `value`, never as it;
@@ -296,9 +300,17 @@ def polymul_leg(V, ctx, ev, pairs_per_prime=256, primes=16, logn=15, reps=5):
     return res
 
 
+def uneven_ops(V):
+    """The five ops of the uneven mix, in draw order: NTT, INTT, polymul,
+    add (the reference's vector add, elemfma with multiplier 1, as a batch
+    job: JOB_ADD_REF), mul."""
+    return [V.JOB_NTT_FWD, V.JOB_NTT_INV, V.JOB_POLYMUL, V.JOB_ADD_REF, V.JOB_MUL]
+
+
 def make_uneven_jobs(V, ctx, njobs=1000, seed=20261202):
     """Uneven batch: n = 2^10..2^17, L = 1..24 primes < 2^60 (q = 1 mod
-    2^18), op uniform over NTT, INTT, polymul, add, mul."""
+    2^18), op uniform over NTT, INTT, polymul, add (elemfma m = 1), mul."""
+    ops = uneven_ops(V)
     rng = np.random.default_rng(seed)
     qs = primes_ntt(1 << 60, 1 << 18, 24)
     tables = {}
@@ -306,7 +318,7 @@ def make_uneven_jobs(V, ctx, njobs=1000, seed=20261202):
     for k in range(njobs):
         logn = int(rng.integers(10, 18))
         L = int(rng.integers(1, 25))
-        op = int(rng.integers(0, 5))
+        op = ops[int(rng.integers(0, 5))]
         n = 1 << logn
         for q in qs[:L]:
             if (n, q) not in tables:
@@ -333,7 +345,7 @@ def job_coeffs(V, job):
 def per_call_service(V, ctx, jobs):
     """Service time (s) of each job through synchronous reference-API calls
     on limb views (NTT/INTT: 1 call per limb; polymul: 4; mul: elemmul;
-    add: elemfma with multiplier 1)."""
+    add: elemfma with multiplier 1, the function JOB_ADD_REF batches)."""
     times = []
     tmp = ctx.vector(1 << 17)
     for op, a, b, r, tabs in jobs:
@@ -357,21 +369,13 @@ def per_call_service(V, ctx, jobs):
                 vr.inverse_transform(vr, t)
             elif op == V.JOB_MUL:
                 va.elemmul(vb, vr, t.q)
-            else:
+            elif op == V.JOB_ADD_REF:
                 va.elemfma(vb, vr, 1, t.q)
+            else:
+                raise ValueError(f"no per-call form for op {op}")
         times.append(time.perf_counter() - t0)
     tmp.close()
     return times
-
-
-def fifo_latencies(arrivals, service):
-    """Latency of each request for one synchronous server taking requests in
-    arrival order (completion_i = max(arrival_i, completion_{i-1}) + s_i)."""
-    done, out = 0.0, []
-    for a, s in zip(arrivals, service):
-        done = max(a, done) + s
-        out.append(done - a)
-    return out
 
 
 def poisson_arrivals(njobs, rate, seed):
@@ -391,7 +395,36 @@ def poisson_arrivals(njobs, rate, seed):
     return out
 
 
-def uneven_and_serving(V, ctx, ev, quick=False):
+def job_sample(job):
+    """The words of a job's last limb (the per-job sample the uneven and
+    serving legs compare)."""
+    op, a, b, r, tabs = job
+    n = tabs[0].n
+    return r.read((len(tabs) - 1) * n, n)
+
+
+def clear_samples(jobs, seed):
+    """Overwrites every job's sampled limb, so a run that skipped a job
+    cannot pass the comparison."""
+    for k, (op, a, b, r, tabs) in enumerate(jobs):
+        n = tabs[0].n
+        r.fill_uniform(3, seed + k, (len(tabs) - 1) * n, n)
+
+
+def samples_equal(jobs, want):
+    return all(np.array_equal(job_sample(j), w) for j, w in zip(jobs, want))
+
+
+SERVE_FRACTIONS = (0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5)
+
+
+def uneven_and_serving(V, ctx, ev, quick=False, repeats=3):
+    """The 1000-job uneven batch (one batch vs the same jobs as per-call
+    reference calls) and its serving at Poisson arrival rates from 10% to
+    150% of the batched job rate, each point `repeats` times, beside a plain
+    per-call server (one request at a time through the synchronous single
+    calls) at the same rates.  Every run is checked against the batch's
+    words on each job's last limb."""
     jobs, tables = make_uneven_jobs(V, ctx, 200 if quick else 1000)
     coeffs = sum(job_coeffs(V, j) for j in jobs)
     bt = V.Batch(ctx)
@@ -399,40 +432,67 @@ def uneven_and_serving(V, ctx, ev, quick=False):
         bt.add(*j)
     batch_ms = timed_launches(ctx, bt, 3, ev)
     bt.close()
+    want = [job_sample(j) for j in jobs]
+    clear_samples(jobs, 1 << 20)
     service = per_call_service(V, ctx, jobs)
+    assert samples_equal(jobs, want), "per-call results differ from the batch's"
     plain_total = sum(service)
     uneven = {"workload": f"{len(jobs)} jobs, n=2^10..2^17, L=1..24 primes <2^60, "
-                          "ops NTT/INTT/polymul/add/mul (seeded)",
+                          "ops NTT/INTT/polymul/add(elemfma m=1)/mul (seeded)",
               "transform_coeffs": coeffs,
               "batch_ms": round(batch_ms, 3),
               "batched_gcoeff_s": round(coeffs / batch_ms / 1e6, 3),
               "plain_per_call_gcoeff_s": round(coeffs / plain_total / 1e9, 4),
               "plain_per_call_ms": round(plain_total * 1e3, 2),
-              "speedup_vs_plain": round(plain_total * 1e3 / batch_ms, 1)}
-    # serving: Poisson arrivals at fractions of the batched job rate
+              "speedup_vs_plain": round(plain_total * 1e3 / batch_ms, 1),
+              "checked": "batch and per-call words equal on every job's last limb"}
     capacity = len(jobs) / (batch_ms * 1e-3)
+    fracs = (0.5,) if quick else SERVE_FRACTIONS
+    reps = 1 if quick else repeats
+
+    def replay(srv, rate, seed):
+        clear_samples(jobs, seed << 12)
+        st, infos = srv.replay(jobs, rate_per_s=rate, seed=seed)
+        assert st == 0, st
+        assert samples_equal(jobs, want), f"served results differ at rate {rate}"
+        lat = [(i.complete_ns - i.submit_ns) / 1e6 for i in infos]
+        span = (max(i.complete_ns for i in infos) -
+                min(i.submit_ns for i in infos)) / 1e9
+        return (pct(lat, 50), pct(lat, 99), coeffs / span / 1e9,
+                float(np.mean([i.batch_size for i in infos])))
+
+    def summary(runs, k):
+        xs = sorted(r[k] for r in runs)
+        return round(xs[len(xs) // 2], 3), [round(xs[0], 3), round(xs[-1], 3)]
+
     points = []
     srv = V.Server(ctx, streams=4)
-    for frac in ([0.5] if quick else [0.1, 0.25, 0.5, 0.75]):
+    plain = V.Server(ctx, per_call=True)
+    for frac in fracs:
         rate = capacity * frac
-        st, infos = srv.replay(jobs, rate_per_s=rate, seed=77)
-        assert st == 0, st
-        lat = [(i.complete_ns - i.submit_ns) / 1e6 for i in infos]
-        span = (max(i.complete_ns for i in infos) - min(i.submit_ns for i in infos)) / 1e9
-        fifo = fifo_latencies(poisson_arrivals(len(jobs), rate, 77), service)
-        points.append({"rate_jobs_s": round(rate, 1),
-                       "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
-                       "gcoeff_s": round(coeffs / span / 1e9, 3),
-                       "mean_batch": round(float(np.mean([i.batch_size for i in infos])), 2),
-                       "plain_fifo_p50_ms": round(pct(fifo, 50) * 1e3, 3),
-                       "plain_fifo_p99_ms": round(pct(fifo, 99) * 1e3, 3)})
+        runs = [replay(srv, rate, 77 + r) for r in range(reps)]
+        pruns = [replay(plain, rate, 177)]
+        p = {"frac_of_capacity": frac, "rate_jobs_s": round(rate, 1), "runs": reps}
+        for k, name in enumerate(("p50_ms", "p99_ms", "gcoeff_s")):
+            p[name], p[name + "_range"] = summary(runs, k)
+        p["mean_batch"] = summary(runs, 3)[0]
+        p["plain_p50_ms"] = round(pruns[0][0], 3)
+        p["plain_p99_ms"] = round(pruns[0][1], 3)
+        p["plain_gcoeff_s"] = round(pruns[0][2], 3)
+        points.append(p)
     st = srv.stats()
     srv.close()
+    plain.close()
     serving = {"streams": 4, "hbm_budget_bytes": 200 * 10**9,
+               "reserved_arena_bytes": int(st.reserved_bytes),
                "batched_capacity_jobs_s": round(capacity, 1),
                "plain_service_p50_ms": round(pct(service, 50) * 1e3, 3),
                "plain_service_p99_ms": round(pct(service, 99) * 1e3, 3),
-               "points": points, "peak_inflight_bytes": int(st.peak_inflight_bytes)}
+               "points": points, "peak_inflight_bytes": int(st.peak_inflight_bytes),
+               "plain": "a served FIFO: vkhel_server per_call mode, one request at a "
+                        "time through the synchronous single calls",
+               "checked": "every replay's words equal the batch's on each job's "
+                          "last limb"}
     for op, a, b, r, tabs in jobs:
         a.close()
         r.close()
@@ -722,29 +782,47 @@ def cpu_polymul_baseline(threads, logn=15, primes=16, target_cpu_s=8.0):
             "one_core_value": round(2 * n / t1 / 1e9, 5)}
 
 
-def plain_polymul_n4096(V, ctx):
+def plain_polymul_n4096(V, ctx, runs=7):
     """One n = 4096 polymul, q the largest 60-bit prime = 1 mod 8192: CPU
-    oracle (one thread) vs the per-call GPU API, bit-exact check."""
+    oracle (one thread) vs the per-call GPU API (forward, forward, elemmul,
+    inverse), each the median of `runs` timed runs after a warm-up run,
+    bit-exact check of every GPU run."""
     from oracle import oracle as O
     n = 4096
     q = primes_ntt(1 << 60, 2 * n, 1)[0]
     w = root_2n(q, n)
     a, b = uniform_mod(q, n, 1), uniform_mod(q, n, 2)
-    t0 = time.perf_counter()
-    want = O.polymul_ntt(a, b, q, w)
-    cpu_s = time.perf_counter() - t0
+    want = O.polymul_ntt(a, b, q, w)  # warm-up
+    cpu = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        got = O.polymul_ntt(a, b, q, w)
+        cpu.append(time.perf_counter() - t0)
+        assert np.array_equal(got, want)
     t = V.NttTables(n, q, w)
-    va, vb, vc = ctx.from_array(a), ctx.from_array(b), ctx.vector(n)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    va.forward_transform(va, t)
-    vb.forward_transform(vb, t)
-    va.elemmul(vb, vc, q)
-    vc.inverse_transform(vc, t)
-    gpu_s = time.perf_counter() - t0
-    ok = bool(np.array_equal(vc.read(), want))
-    return {"q": q, "psi": w, "cpu_oracle_ms": round(cpu_s * 1e3, 3),
-            "gpu_per_call_ms": round(gpu_s * 1e3, 3), "bit_exact": ok}
+    va, vb, vc = ctx.vector(n, False), ctx.vector(n, False), ctx.vector(n)
+    ok, gpu = True, []
+    for k in range(runs + 1):
+        va.copy_from_host(a)
+        vb.copy_from_host(b)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        va.forward_transform(va, t)
+        vb.forward_transform(vb, t)
+        va.elemmul(vb, vc, q)
+        vc.inverse_transform(vc, t)
+        dt = time.perf_counter() - t0
+        if k:  # the first run is the warm-up
+            gpu.append(dt)
+        ok = ok and bool(np.array_equal(vc.read(), want))
+    for v in (va, vb, vc):
+        v.close()
+    t.close()
+    return {"q": q, "psi": w, "runs": runs,
+            "cpu_oracle_ms": round(sorted(cpu)[runs // 2] * 1e3, 3),
+            "gpu_per_call_ms": round(sorted(gpu)[runs // 2] * 1e3, 3),
+            "timing": f"median of {runs} runs after a warm-up run, each side",
+            "bit_exact": ok}
 
 
 def traffic_from_profile(logn, npoly, tile_bits):

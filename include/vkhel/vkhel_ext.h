@@ -146,7 +146,10 @@ int vkhel_ext_autotune_load(struct vkhel_ctx *, const char *path);
  * captured hipGraph when it has more than a few launches).  A job works on an RNS polynomial: `nlimbs` limbs of n words
  * stored back to back in one vector (limb i at word offset i*n), limb i
  * reduced modulo tables[i]->q.  Every tables[i] of one job has the same n.
- * Vectors must hold at least nlimbs*n words.  Inputs of NTT jobs follow the
+ * Vectors must hold at least nlimbs*n words.  The vectors and tables a job
+ * names must outlive every launch of the batch that holds it (destroying
+ * tables waits for the device, but a launch enqueued later would read
+ * freed twiddles).  Inputs of NTT jobs follow the
  * single-call rules: forward accepts any 64-bit words, inverse and polymul
  * operands of the inverse step expect words < q.
  */
@@ -183,6 +186,13 @@ enum vkhel_job_op {
 	 * their forward transforms and one pointwise pass, after every other
 	 * transform of the batch.  Same checks as VKHEL_JOB_RESCALE. */
 	VKHEL_JOB_RESCALE_NTT = 10,
+	/* res = elemfma(a, b, 1, q) per limb: the reference's vector add
+	 * (src/vector.c:298-340, elemfma.comp:36-55 with multiplier 1) word for
+	 * word, including its q - x correction that wraps modulo 2^64 where
+	 * (a mod q) + b >= q; any 64-bit inputs.  A batch of these computes
+	 * exactly what one vkhel_vector_elemfma(a, b, res, 1, q) call per limb
+	 * does; VKHEL_JOB_ADD is the correct (a + b) mod q. */
+	VKHEL_JOB_ADD_REF = 11,
 };
 
 struct vkhel_batch;
@@ -223,15 +233,19 @@ uint64_t vkhel_batch_num_jobs(const struct vkhel_batch *);
 
 /* ---- serving engine ----
  * Continuous batching over device-resident operands.  Requests use the
- * batch job model.  They are queued on submit and admitted in arrival order
- * while the device bytes of admitted, unfinished requests (operands, result
- * and polymul scratch) stay within hbm_budget_bytes and at most
- * max_inflight requests are in flight.  `streams` workers each own a HIP
- * stream; a free worker takes every admissible waiting request (up to
- * max_batch) as one micro-batch, so requests arriving while others run are
- * admitted as budget and slots free up and overlap with them on the device.
- * A request's result does not depend on what it was batched with.  Callers
- * keep operands alive and unmodified until the request completes.
+ * batch job model.  `streams` workers each own a HIP stream, task arrays
+ * and a scratch arena (polymul spectra, rescaling temporaries), all
+ * allocated at vkhel_server_create; the arenas count against
+ * hbm_budget_bytes and nothing is allocated while serving.  Requests are
+ * queued on submit and admitted in arrival order while the operand and
+ * result bytes of admitted, unfinished requests stay within the budget
+ * left after the arenas and at most max_inflight requests are in flight.
+ * A free worker takes every admissible waiting request that fits its arena
+ * and task arrays (up to max_batch) as one micro-batch, so requests
+ * arriving while others run are admitted as budget and slots free up and
+ * overlap with them on the device.  A request's result does not depend on
+ * what it was batched with.  Callers keep operands alive and unmodified,
+ * and their tables alive, until the request completes.
  */
 struct vkhel_server_config {
 	uint64_t hbm_budget_bytes; /* 0: 200 GB */
@@ -239,6 +253,15 @@ struct vkhel_server_config {
 	uint32_t max_batch;        /* requests per micro-batch, 0: 256 */
 	uint32_t streams;          /* worker streams, 0: 4 (at most 32) */
 	uint32_t queue_capacity;   /* waiting requests, 0: 65536 */
+	/* scratch arena per stream; 0: 1 GiB, or hbm_budget_bytes / 8 /
+	 * streams when that is less */
+	uint64_t scratch_bytes;
+	/* polynomial limbs per micro-batch (task array capacity), 0: 16384 */
+	uint32_t task_limbs;
+	/* 1: plain per-call service instead of batching -- one stream, one
+	 * request at a time in arrival order, each limb and step a synchronous
+	 * single call (the comparison point for latency); 0: batching */
+	uint32_t per_call;
 };
 
 struct vkhel_request_info {
@@ -255,6 +278,7 @@ struct vkhel_server_stats {
 	uint64_t transform_coeffs;   /* NTT coefficients processed */
 	uint64_t inflight_bytes, peak_inflight_bytes;
 	uint32_t inflight, waiting;
+	uint64_t reserved_bytes;     /* scratch arenas held since creation */
 };
 
 struct vkhel_server;
@@ -263,8 +287,9 @@ struct vkhel_server *vkhel_server_create(struct vkhel_ctx *,
 /* Finishes admitted and waiting work, then stops the workers. */
 void vkhel_server_destroy(struct vkhel_server *);
 /* Returns a request id > 0, or 0 with the error recorded: VKHEL_ERR_BUDGET
- * if the request alone exceeds the budget, VKHEL_ERR_BUSY if the queue is
- * full, VKHEL_ERR_INVALID for bad arguments. */
+ * if the request alone exceeds the operand budget, a worker's arena or its
+ * task arrays, VKHEL_ERR_BUSY if the queue is full, VKHEL_ERR_INVALID for
+ * bad arguments. */
 uint64_t vkhel_server_submit(struct vkhel_server *, int op,
 		const struct vkhel_vector *a, const struct vkhel_vector *b,
 		struct vkhel_vector *res, struct vkhel_ntt_tables *const *tables,

@@ -34,19 +34,24 @@ def tables_for(logn, nlimbs):
     return _TABLE_CACHE[key]
 
 
-def make_jobs(ctx, rng, count, logn_range=(10, 15), max_limbs=4):
+def make_jobs(ctx, rng, count, logn_range=(10, 15), max_limbs=4, ops=None):
+    """Random jobs; op drawn from NTT / INTT / polymul / mul / add, or
+    ops[k % len(ops)] for job k when `ops` is given."""
     jobs = []
     for k in range(count):
         logn = int(rng.integers(*logn_range))
         L = int(rng.integers(1, max_limbs + 1))
         tabs = tables_for(logn, L)
         n = 1 << logn
-        op = int(rng.integers(0, 5))
-        a, b, r = ctx.vector(L * n, False), ctx.vector(L * n, False), ctx.vector(L * n)
+        op = int(rng.integers(0, 5)) if ops is None else ops[k % len(ops)]
+        binary = op >= V.JOB_POLYMUL
+        a, r = ctx.vector(L * n, False), ctx.vector(L * n)
+        b = ctx.vector(L * n, False) if binary else None
         for i, t in enumerate(tabs):
             a.fill_uniform(t.q, 1000 * k + i, i * n, n)
-            b.fill_uniform(t.q, 1000 * k + 500 + i, i * n, n)
-        jobs.append((op, a, b if op >= V.JOB_POLYMUL else None, r, tabs))
+            if binary:
+                b.fill_uniform(t.q, 1000 * k + 500 + i, i * n, n)
+        jobs.append((op, a, b, r, tabs))
     return jobs
 
 
@@ -118,17 +123,74 @@ def test_poisson_arrivals_latency_records(ctx):
 def test_budget_limits_inflight_bytes(ctx):
     rng = np.random.default_rng(13)
     jobs = make_jobs(ctx, rng, 24, (12, 14), 2)
-    # admission cost: operands + result (+ the spectrum of b for polymul)
-    def cost(op, b, words):
-        return 8 * words * ((3 if b is not None else 2) + (op == V.JOB_POLYMUL))
-    biggest = max(cost(op, b, r.length) for op, a, b, r, tabs in jobs)
-    budget = int(biggest * 2)
-    srv = V.Server(ctx, hbm_budget_bytes=budget, streams=4)
+    # admission cost: operands + result; the workers' scratch arenas are
+    # reserved from the budget when the server is created
+    def cost(b, words):
+        return 8 * words * (3 if b is not None else 2)
+    biggest = max(cost(b, r.length) for op, a, b, r, tabs in jobs)
+    arena = 1 << 20
+    budget = 4 * arena + int(biggest * 2)
+    srv = V.Server(ctx, hbm_budget_bytes=budget, streams=4, scratch_bytes=arena)
     st, infos = srv.replay(jobs, 0)
     assert st == VV.OK
     s = srv.stats()
-    assert 0 < s.peak_inflight_bytes <= budget
+    assert s.reserved_bytes == 4 * arena
+    assert 0 < s.peak_inflight_bytes <= budget - s.reserved_bytes
     srv.close()
+
+
+def test_arenas_bound_micro_batches_and_requests(ctx):
+    tabs = tables_for(13, 2)
+    n = 1 << 13
+    words = 2 * n
+    vecs, jobs = [], []
+    for k in range(6):
+        a, b, r = (ctx.vector(words, False) for _ in range(3))
+        for i, t in enumerate(tabs):
+            a.fill_uniform(t.q, 40 + 10 * k + i, i * n, n)
+            b.fill_uniform(t.q, 45 + 10 * k + i, i * n, n)
+        vecs += [a, b, r]
+        jobs.append((V.JOB_POLYMUL, a, b, r, tabs))
+    # an arena that holds exactly one product's spectra (2 limbs x n words)
+    srv = V.Server(ctx, streams=2, scratch_bytes=8 * words)
+    st, infos = srv.replay(jobs, 0)
+    assert st == VV.OK
+    assert max(i.batch_size for i in infos) == 1
+    for op, a, b, r, t in jobs:
+        assert np.array_equal(r.read(), run_alone(ctx, (op, a, b, r, t)))
+    srv.close()
+    # a product whose spectra exceed one arena is refused at submit
+    small = V.Server(ctx, streams=2, scratch_bytes=8 * n)
+    with pytest.raises(V.VkhelError) as e:
+        small.submit(*jobs[0])
+    assert e.value.code == VV.ERR_BUDGET
+    small.close()
+    # arenas larger than the budget: the server is not created
+    with pytest.raises(V.VkhelError) as e:
+        V.Server(ctx, hbm_budget_bytes=1 << 20, streams=4, scratch_bytes=1 << 20)
+    assert e.value.code == VV.ERR_BUDGET
+    for v in vecs:
+        v.close()
+
+
+def test_per_call_server_matches_batch(ctx):
+    """The plain per-call mode serves every request alone through the
+    synchronous single calls; its words equal the batch path's."""
+    rng = np.random.default_rng(19)
+    ops = [V.JOB_NTT_FWD, V.JOB_NTT_INV, V.JOB_POLYMUL, V.JOB_ADD, V.JOB_MUL,
+           V.JOB_ADD_REF]
+    jobs = make_jobs(ctx, rng, 30, (10, 14), 3, ops=ops)
+    srv = V.Server(ctx, per_call=True, streams=4, max_batch=64)
+    st, infos = srv.replay(jobs, 50000, seed=3)
+    assert st == VV.OK
+    assert all(i.batch_size == 1 for i in infos)
+    for job in jobs:
+        assert np.array_equal(job[3].read(), run_alone(ctx, job)), job[0]
+    srv.close()
+    for _, a, b, r, _ in jobs:
+        for v in (a, b, r):
+            if v is not None:
+                v.close()
 
 
 def test_request_over_budget_rejected(ctx):

@@ -40,6 +40,8 @@ __device__ __forceinline__ uint64_t point_op(const PointTask &t, uint64_t x,
 	}
 	case kPtAdd:
 		return csub(x + y, t.q);
+	case kPtFma:
+		return elemfma_word(x, y, t.w, t.wp, t.q);
 	case kPtGtSub: {  // elemgtsub.comp:49-67
 		const uint64_t r = barrett64(x, t.q, t.mu, t.shift);
 		if (x > t.bound)
@@ -65,7 +67,8 @@ __device__ __forceinline__ uint64_t point_op(const PointTask &t, uint64_t x,
 // One workgroup row per task (blockIdx.y); unary ops never read b.
 __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks) {
 	const PointTask t = tasks[blockIdx.y];
-	const bool binary = t.op == kPtMul || t.op == kPtAdd || t.op == kPtRescale;
+	const bool binary = t.op == kPtMul || t.op == kPtAdd || t.op == kPtRescale ||
+		t.op == kPtFma;
 	const uint64_t pairs = t.len >> 1;
 	const uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
 	for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -99,12 +102,36 @@ void DeviceTasks::release() {
 	cap_pt = cap_post = cap_post2 = 0;
 }
 
+template <class T>
+static bool reserve_array(T **d, size_t *cap, size_t n) {
+	if (*cap >= n)
+		return true;
+	(void) hipFree(*d);
+	*d = nullptr;
+	*cap = 0;
+	if (!hip_ok(hipMalloc(d, n * sizeof(T)), "hipMalloc(task array)"))
+		return false;
+	*cap = n;
+	return true;
+}
+
+bool DeviceTasks::reserve(size_t n) {
+	return reserve_array(&fwd, &cap_fwd, n) && reserve_array(&inv, &cap_inv, n) &&
+		reserve_array(&pm_col, &cap_pm_col, n) && reserve_array(&pm, &cap_pm, n) &&
+		reserve_array(&fwd2, &cap_fwd2, n) && reserve_array(&pt, &cap_pt, n) &&
+		reserve_array(&post, &cap_post, n) && reserve_array(&post2, &cap_post2, n);
+}
+
+uint64_t DeviceTasks::bytes_for(size_t n) {
+	return (uint64_t) n * (5 * sizeof(NttTask) + 3 * sizeof(PointTask));
+}
+
 int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 		const vkhel_vector *b, vkhel_vector *res,
 		vkhel_ntt_tables *const *tables, uint32_t nlimbs, Job *out) {
 	if (!a || !res || !tables || nlimbs == 0)
 		return fail(VKHEL_ERR_INVALID, "job: null argument");
-	if (op < VKHEL_JOB_NTT_FWD || op > VKHEL_JOB_ADD)
+	if ((op < VKHEL_JOB_NTT_FWD || op > VKHEL_JOB_ADD) && op != VKHEL_JOB_ADD_REF)
 		return fail(VKHEL_ERR_INVALID, "job: unknown op %d", op);
 	const bool binary = op >= VKHEL_JOB_POLYMUL;
 	if (binary && !b)
@@ -194,10 +221,10 @@ uint64_t job_scratch_words(const Job &j) {
 	return j.op == VKHEL_JOB_POLYMUL ? j.tables.size() * j.tables[0]->n : 0;
 }
 
-uint64_t job_bytes(const Job &j) {
+uint64_t job_operand_bytes(const Job &j) {
 	const uint64_t words = j.tables.size() * j.tables[0]->n;
 	const uint64_t operands = j.b ? 3 : 2;
-	return 8 * (operands * words + job_scratch_words(j));
+	return 8 * operands * words;
 }
 
 uint64_t job_transform_coeffs(const Job &j) {
@@ -391,12 +418,19 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				break;
 			}
 			case VKHEL_JOB_MUL:
-			case VKHEL_JOB_ADD: {
+			case VKHEL_JOB_ADD:
+			case VKHEL_JOB_ADD_REF: {
 				PointTask p = mul;
 				p.a = j.a->dev + off;
 				p.b = j.b->dev + off;
 				p.out = j.res->dev + off;
-				p.op = j.op == VKHEL_JOB_MUL ? kPtMul : kPtAdd;
+				p.op = j.op == VKHEL_JOB_MUL ? kPtMul
+					: j.op == VKHEL_JOB_ADD ? kPtAdd : kPtFma;
+				if (p.op == kPtFma) {
+					// elemfma with multiplier 1 (src/vector.c:298-340)
+					p.w = 1;
+					p.wp = barrett_factor(1, q);
+				}
 				ts.pt.push_back(p);
 				break;
 			}

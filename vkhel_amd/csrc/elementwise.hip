@@ -24,11 +24,7 @@ struct OpMul {
 struct OpFma {
 	uint64_t q, m, mp;
 	__device__ uint64_t operator()(uint64_t x, uint64_t y) const {
-		uint64_t p = x * m - mulhi64(x, mp) * q;
-		if (p >= q)
-			p = q - p;
-		const uint64_t s = p + y;
-		return s >= q ? q - s : s;
+		return elemfma_word(x, y, m, mp, q);
 	}
 };
 
@@ -150,6 +146,11 @@ int launch_addmod(const uint64_t *a, const uint64_t *b, uint64_t *out,
 	return launch_binary(a, b, out, len, OpAddMod{q}, s);
 }
 
+int launch_elemfma(const uint64_t *a, const uint64_t *b, uint64_t *out,
+		uint64_t len, uint64_t m, uint64_t q, hipStream_t s) {
+	return launch_binary(a, b, out, len, OpFma{q, m, barrett_factor(m, q)}, s);
+}
+
 // Shared argument checks: same context (src/vector.c:302, 345, 392, 432,
 // 474) and inputs at least as long as the result, whose length sets the
 // dispatch size (src/kernels/elemmul.c:164).
@@ -188,9 +189,8 @@ void vkhel_vector_elemfma(const struct vkhel_vector *a,
 		return;
 	if (mod < 2)
 		return (void) fail(VKHEL_ERR_INVALID, "elemfma: mod must be >= 2");
-	const OpFma op{mod, multiplier, barrett_factor(multiplier, mod)};
-	finish(res->ctx, launch_binary(a->dev, b->dev, res->dev, res->length, op,
-				res->ctx->stream));
+	finish(res->ctx, launch_elemfma(a->dev, b->dev, res->dev, res->length,
+				multiplier, mod, res->ctx->stream));
 }
 
 // src/vector.c:342-386: mod 2 -> elemmodbytwo(q/2), else elemgtsub(q/2, q).

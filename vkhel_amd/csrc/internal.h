@@ -219,5 +219,8 @@ int launch_elemmul(const uint64_t *a, const uint64_t *b, uint64_t *out,
 		uint64_t len, uint64_t q, hipStream_t s);
 int launch_addmod(const uint64_t *a, const uint64_t *b, uint64_t *out,
 		uint64_t len, uint64_t q, hipStream_t s);
+// vkhel_vector_elemfma's kernel (elemfma.comp:36-55), stream-ordered.
+int launch_elemfma(const uint64_t *a, const uint64_t *b, uint64_t *out,
+		uint64_t len, uint64_t m, uint64_t q, hipStream_t s);
 
 }  // namespace vk

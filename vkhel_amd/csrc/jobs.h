@@ -23,6 +23,7 @@ struct Job {
 // One pointwise limb over len words.  op:
 //   kPtMul   out = a.b mod q (elemmul semantics)
 //   kPtAdd   out = (a + b) mod q
+//   kPtFma   out = elemfma(a, b, w, q), wp = floor(w * 2^64 / q)
 //   kPtGtSub out = elemgtsub(a, bound, diff, q)  (also elemmod, p != 2)
 //   kPtGtAdd out = elemgtadd(a, bound, diff)
 //   kPtModTwo out = elemmod(a, 2, .) with bound = q_limb / 2
@@ -30,7 +31,7 @@ struct Job {
 //              dropped limb b (b > bound: b - q_last) reduced mod q,
 //              diff = q_last mod q, w = q_last^-1 mod q (wp its packed
 //              Shoup companion)
-enum { kPtMul = 0, kPtAdd, kPtGtSub, kPtGtAdd, kPtModTwo, kPtRescale };
+enum { kPtMul = 0, kPtAdd, kPtGtSub, kPtGtAdd, kPtModTwo, kPtRescale, kPtFma };
 struct PointTask {
 	const uint64_t *a, *b;
 	uint64_t *out;
@@ -74,6 +75,11 @@ struct DeviceTasks {
 	size_t cap_fwd = 0, cap_inv = 0, cap_pm_col = 0, cap_pm = 0, cap_fwd2 = 0;
 	size_t cap_pt = 0, cap_post = 0, cap_post2 = 0;
 	void release();
+	// Allocates every array for `tasks` entries up front (false with the
+	// error recorded on failure).
+	bool reserve(size_t tasks);
+	// Device bytes `reserve(tasks)` takes.
+	static uint64_t bytes_for(size_t tasks);
 };
 
 // Checks a job's arguments and fills `out`.  Returns a vkhel_status.
@@ -88,9 +94,9 @@ int make_rns_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
 // Device scratch words a job needs (the spectrum of b for POLYMUL, the
 // dropped limb's inverse transform for RESCALE_NTT).
 uint64_t job_scratch_words(const Job &j);
-// Device bytes a job touches (operands, result, scratch): the serving
-// engine's admission cost.
-uint64_t job_bytes(const Job &j);
+// Device bytes of a job's operands and result: the serving engine's
+// admission cost (its scratch comes from the engine's reserved arenas).
+uint64_t job_operand_bytes(const Job &j);
 // Transform coefficients a job processes (for throughput accounting).
 uint64_t job_transform_coeffs(const Job &j);
 // Plans `jobs` into tasks; POLYMUL scratch is carved from `scratch`.

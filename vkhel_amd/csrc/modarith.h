@@ -110,6 +110,20 @@ __device__ __forceinline__ uint64_t shoup63_lazy(uint64_t y, uint64_t w,
 	return shoup63_impl<false>(y, w, wpk, nq);
 }
 
+// elemfma.comp:36-55 on one word: x*m mod q by the Shoup factor
+// mp = floor(m * 2^64 / q) with its correction written `q - p` (which wraps
+// modulo 2^64 where p - q was meant), then the sum with y corrected the same
+// way (src/kernels/elemfma.c:159-169 builds m and mp).  Kept exactly, so the
+// result equals the reference's on every input.
+__device__ __forceinline__ uint64_t elemfma_word(uint64_t x, uint64_t y,
+		uint64_t m, uint64_t mp, uint64_t q) {
+	uint64_t p = x * m - __umul64hi(x, mp) * q;
+	if (p >= q)
+		p = q - p;
+	const uint64_t s = p + y;
+	return s >= q ? q - s : s;
+}
+
 // Reference 64-bit Barrett (elemmul.comp:36-47, elemgtsub.comp:36-47):
 // mu = floor(2^(bits+62) / q), shift = bits - 2, one subtraction.  Exact
 // (canonical) for every 64-bit x once bits >= 3.

@@ -257,7 +257,10 @@ struct vkhel_ntt_tables *vkhel_ntt_tables_create(uint64_t n, uint64_t q,
 }
 
 // src/ntt_tables.c:82-87 (which leaks the struct; this frees it) plus the
-// device copies in every live context.
+// device copies in every live context.  A device copy may be in use on any
+// stream of the device (batch, serving workers, callers' streams), so the
+// whole device is waited for before it is freed; callers must still not
+// destroy tables that a queued batch launch or server request names.
 void vkhel_ntt_tables_destroy(struct vkhel_ntt_tables *t) {
 	if (!t)
 		return;
@@ -267,7 +270,8 @@ void vkhel_ntt_tables_destroy(struct vkhel_ntt_tables *t) {
 			std::lock_guard<std::mutex> g2(ctx->mu);
 			auto it = ctx->tables.find(t->id);
 			if (it != ctx->tables.end()) {
-				(void) hipStreamSynchronize(ctx->stream);
+				(void) hipSetDevice(ctx->device);
+				(void) hipDeviceSynchronize();
 				free_dev_tables(it->second);
 				ctx->tables.erase(it);
 			}

@@ -6,9 +6,11 @@
 // Requests are queued on submit and admitted in arrival order while the
 // device footprint of admitted-but-unfinished requests stays within the
 // HBM budget and the in-flight count within its limit.  `streams` worker
-// threads each own a HIP stream, a task-array buffer and a scratch arena;
-// a worker that becomes free takes every admissible waiting request (up to
-// max_batch) as one micro-batch, plans it into the batch executor's three
+// threads each own a HIP stream, task arrays and a scratch arena, all
+// allocated when the server is created (the arenas count against the
+// budget; nothing is allocated while serving); a worker that becomes free
+// takes every admissible waiting request that fits its arena (up to
+// max_batch) as one micro-batch, plans it into the batch executor's
 // phases and runs it.  Requests that arrive while other micro-batches are
 // on the device are therefore admitted as budget and slots free up, and
 // their kernels overlap with the running ones on the other streams.  Each
@@ -34,7 +36,9 @@ enum ReqState { kWaiting, kRunning, kDone };
 
 struct Request {
 	Job job;
-	uint64_t bytes = 0;
+	uint64_t bytes = 0;    // operands and result
+	uint64_t scratch = 0;  // words of the worker's arena it needs
+	uint64_t limbs = 0;
 	int state = kWaiting;
 	int status = VKHEL_OK;
 	vkhel_request_info info{};
@@ -58,6 +62,11 @@ struct vkhel_server {
 	std::deque<uint64_t> waiting;
 	std::unordered_map<uint64_t, vk::Request> reqs;
 	uint64_t next_id = 1;
+	// device bytes of the workers' scratch arenas (held from creation),
+	// and the per-worker arena and task capacity in words / limbs
+	uint64_t reserved_bytes = 0;
+	uint64_t arena_words = 0;
+	uint64_t task_limbs = 0;
 	uint64_t inflight_bytes = 0;
 	uint32_t inflight = 0;
 	bool stop = false;
@@ -68,28 +77,24 @@ struct vkhel_server {
 
 namespace vk {
 
+// Bytes of the budget left for requests' operands once the arenas are
+// reserved.
+static uint64_t operand_budget(const vkhel_server *s) {
+	return s->cfg.hbm_budget_bytes - s->reserved_bytes;
+}
+
 // True when the oldest waiting request can be admitted now.
 static bool head_admissible(const vkhel_server *s) {
 	if (s->waiting.empty())
 		return false;
 	const Request &r = s->reqs.at(s->waiting.front());
 	return s->inflight < s->cfg.max_inflight &&
-		s->inflight_bytes + r.bytes <= s->cfg.hbm_budget_bytes;
+		s->inflight_bytes + r.bytes <= operand_budget(s);
 }
 
 static int run_micro_batch(vkhel_server *s, Worker &w,
 		const std::vector<const Job *> &jobs) {
-	uint64_t scratch = 0;
-	for (const Job *j : jobs)
-		scratch += job_scratch_words(*j);
-	if (scratch > w.scratch_words) {
-		(void) hipFree(w.scratch);
-		w.scratch = nullptr;
-		w.scratch_words = 0;
-		if (!hip_ok(hipMalloc(&w.scratch, scratch * 8), "hipMalloc(serve scratch)"))
-			return vkhel_ext_last_error();
-		w.scratch_words = scratch;
-	}
+	// admission kept the batch within the worker's arena and task arrays
 	int rc = plan_tasks(s->ctx, jobs.data(), jobs.size(), w.scratch, w.ts);
 	if (rc == VKHEL_OK)
 		rc = upload_tasks(w.ts, w.dt, w.stream);
@@ -98,6 +103,63 @@ static int run_micro_batch(vkhel_server *s, Worker &w,
 	if (rc == VKHEL_OK && !hip_ok(hipStreamSynchronize(w.stream), "serve sync"))
 		rc = VKHEL_ERR_DEVICE;
 	return rc;
+}
+
+// Plain per-call service of one request (vkhel_server_config.per_call): the
+// library's synchronous single calls, one per limb and step, each waited
+// for before the next as a reference caller does (src/vector.c:320-334
+// fences every call): NTT / INTT one transform call, POLYMUL two forward
+// transforms, elemmul and the inverse, MUL elemmul, ADD the modular add,
+// ADD_REF elemfma with multiplier 1.  Other ops run as a one-job batch.
+static int run_per_call(vkhel_server *s, Worker &w, const Job &j) {
+	if (j.op > VKHEL_JOB_ADD && j.op != VKHEL_JOB_ADD_REF) {
+		const Job *one = &j;
+		return run_micro_batch(s, w, std::vector<const Job *>(1, one));
+	}
+	const uint64_t n = j.tables[0]->n;
+	for (size_t l = 0; l < j.tables.size(); l++) {
+		const uint64_t off = l * n, q = j.tables[l]->q;
+		const uint64_t *a = j.a->dev + off;
+		const uint64_t *b = j.b ? j.b->dev + off : nullptr;
+		uint64_t *r = j.res->dev + off;
+		const DevTables *t = nullptr;
+		if (j.op <= VKHEL_JOB_POLYMUL && !(t = device_tables(s->ctx, j.tables[l])))
+			return vkhel_ext_last_error();
+		auto call = [&](int rc) {
+			if (rc == VKHEL_OK && !hip_ok(hipStreamSynchronize(w.stream), "sync"))
+				rc = VKHEL_ERR_DEVICE;
+			return rc;
+		};
+		int rc = VKHEL_OK;
+		switch (j.op) {
+		case VKHEL_JOB_NTT_FWD:
+		case VKHEL_JOB_NTT_INV:
+			rc = call(launch_ntt(s->ctx, *t, a, r, 1, j.op == VKHEL_JOB_NTT_INV,
+						w.stream));
+			break;
+		case VKHEL_JOB_POLYMUL:
+			rc = call(launch_ntt(s->ctx, *t, a, r, 1, false, w.stream));
+			if (rc == VKHEL_OK)
+				rc = call(launch_ntt(s->ctx, *t, b, w.scratch, 1, false, w.stream));
+			if (rc == VKHEL_OK)
+				rc = call(launch_elemmul(r, w.scratch, r, n, q, w.stream));
+			if (rc == VKHEL_OK)
+				rc = call(launch_ntt(s->ctx, *t, r, r, 1, true, w.stream));
+			break;
+		case VKHEL_JOB_MUL:
+			rc = call(launch_elemmul(a, b, r, n, q, w.stream));
+			break;
+		case VKHEL_JOB_ADD:
+			rc = call(launch_addmod(a, b, r, n, q, w.stream));
+			break;
+		default:
+			rc = call(launch_elemfma(a, b, r, n, 1, q, w.stream));
+			break;
+		}
+		if (rc != VKHEL_OK)
+			return rc;
+	}
+	return VKHEL_OK;
 }
 
 static void worker_loop(vkhel_server *s, int wi) {
@@ -114,10 +176,18 @@ static void worker_loop(vkhel_server *s, int wi) {
 			if (s->waiting.empty() && s->stop)
 				return;
 			const uint64_t t = now_ns();
+			uint64_t arena = 0, limbs = 0;
 			while (ids.size() < s->cfg.max_batch && head_admissible(s)) {
 				const uint64_t id = s->waiting.front();
-				s->waiting.pop_front();
 				Request &r = s->reqs.at(id);
+				// the micro-batch must fit this worker's arena and task
+				// arrays (a request alone always does: submit checks it)
+				if (arena + r.scratch > s->arena_words ||
+						limbs + r.limbs > s->task_limbs)
+					break;
+				arena += r.scratch;
+				limbs += r.limbs;
+				s->waiting.pop_front();
 				r.state = kRunning;
 				r.info.admit_ns = t;
 				s->inflight++;
@@ -132,7 +202,8 @@ static void worker_loop(vkhel_server *s, int wi) {
 			continue;
 		// Job records stay valid: only the waiter erases a request, and only
 		// once it is done.
-		const int rc = run_micro_batch(s, w, jobs);
+		const int rc = s->cfg.per_call ? run_per_call(s, w, *jobs[0])
+			: run_micro_batch(s, w, jobs);
 		const uint64_t t = now_ns();
 		{
 			std::lock_guard<std::mutex> lk(s->mu);
@@ -181,18 +252,52 @@ struct vkhel_server *vkhel_server_create(struct vkhel_ctx *ctx,
 		s->cfg.streams = 4;
 	if (!s->cfg.queue_capacity)
 		s->cfg.queue_capacity = 65536;
+	if (s->cfg.per_call) {
+		// one request at a time, in arrival order: a plain synchronous server
+		s->cfg.streams = 1;
+		s->cfg.max_batch = 1;
+	}
 	if (s->cfg.streams > 32) {
 		delete s;
 		return fail(VKHEL_ERR_INVALID, "server_create: at most 32 streams"),
 			nullptr;
 	}
+	// Arenas: 1 GiB per stream by default, or an eighth of the budget over
+	// the streams when that is less; they count against the budget.
+	if (!s->cfg.scratch_bytes)
+		s->cfg.scratch_bytes = std::min<uint64_t>(1ull << 30,
+				s->cfg.hbm_budget_bytes / 8 / s->cfg.streams);
+	s->arena_words = s->cfg.scratch_bytes / 8;
+	s->reserved_bytes = 8 * s->arena_words * s->cfg.streams;
+	if (s->reserved_bytes > s->cfg.hbm_budget_bytes) {
+		delete s;
+		return fail(VKHEL_ERR_BUDGET, "server_create: %u arenas of %llu bytes "
+				"exceed the budget of %llu bytes", s->cfg.streams,
+				(unsigned long long) s->cfg.scratch_bytes,
+				(unsigned long long) s->cfg.hbm_budget_bytes), nullptr;
+	}
+	if (!s->cfg.task_limbs)
+		s->cfg.task_limbs = 16384;
+	s->task_limbs = std::max<uint64_t>(s->cfg.task_limbs, 1);
 	s->workers.resize(s->cfg.streams);
 	for (Worker &w : s->workers) {
-		if (!hip_ok(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking),
-					"server stream")) {
-			for (Worker &x : s->workers)
+		// every task array holds two tasks per limb (a polymul limb's two
+		// forward transforms); RESCALE_NTT needs one per array and limb
+		bool ok = hip_ok(hipStreamCreateWithFlags(&w.stream,
+					hipStreamNonBlocking), "server stream") &&
+			w.dt.reserve(2 * s->task_limbs);
+		if (ok && s->arena_words) {
+			ok = hip_ok(hipMalloc(&w.scratch, 8 * s->arena_words),
+					"hipMalloc(server arena)");
+			w.scratch_words = ok ? s->arena_words : 0;
+		}
+		if (!ok) {
+			for (Worker &x : s->workers) {
 				if (x.stream)
 					(void) hipStreamDestroy(x.stream);
+				x.dt.release();
+				(void) hipFree(x.scratch);
+			}
 			delete s;
 			return nullptr;
 		}
@@ -232,19 +337,33 @@ void vkhel_server_destroy(struct vkhel_server *s) {
 
 // Queues a checked job; returns its id, or 0 with the error recorded.
 static uint64_t submit_request(vkhel_server *s, Request &&r) {
-	r.bytes = job_bytes(r.job);
+	r.bytes = job_operand_bytes(r.job);
+	r.scratch = job_scratch_words(r.job);
+	r.limbs = r.job.tables.size();
 	r.info.submit_ns = now_ns();
 	uint64_t id;
 	{
 		std::lock_guard<std::mutex> lk(s->mu);
 		if (s->stop)
 			return fail(VKHEL_ERR_INVALID, "server_submit: server stopping"), 0;
-		if (r.bytes > s->cfg.hbm_budget_bytes) {
+		if (r.bytes > operand_budget(s)) {
 			s->stats.rejected++;
 			return fail(VKHEL_ERR_BUDGET,
-					"server_submit: request needs %llu bytes, budget is %llu",
+					"server_submit: request needs %llu bytes, budget is %llu "
+					"(%llu after the %llu reserved for scratch arenas)",
 					(unsigned long long) r.bytes,
-					(unsigned long long) s->cfg.hbm_budget_bytes), 0;
+					(unsigned long long) s->cfg.hbm_budget_bytes,
+					(unsigned long long) operand_budget(s),
+					(unsigned long long) s->reserved_bytes), 0;
+		}
+		if (r.scratch > s->arena_words || r.limbs > s->task_limbs) {
+			s->stats.rejected++;
+			return fail(VKHEL_ERR_BUDGET,
+					"server_submit: request needs %llu scratch words and %llu "
+					"limbs; a worker holds %llu and %llu",
+					(unsigned long long) r.scratch, (unsigned long long) r.limbs,
+					(unsigned long long) s->arena_words,
+					(unsigned long long) s->task_limbs), 0;
 		}
 		if (s->waiting.size() >= s->cfg.queue_capacity) {
 			s->stats.rejected++;
@@ -343,6 +462,7 @@ int vkhel_server_get_stats(struct vkhel_server *s, struct vkhel_server_stats *ou
 		return fail(VKHEL_ERR_INVALID, "server_stats: null argument");
 	std::lock_guard<std::mutex> lk(s->mu);
 	*out = s->stats;
+	out->reserved_bytes = s->reserved_bytes;
 	out->inflight_bytes = s->inflight_bytes;
 	out->inflight = s->inflight;
 	out->waiting = (uint32_t) s->waiting.size();

@@ -18,6 +18,7 @@ JOB_CENTER, JOB_GTADD, JOB_GTSUB = 5, 6, 7  # RNS elementwise (add_rns)
 JOB_INTT_CENTER = 8  # fused INTT + centered lift (add_rns, params [p])
 JOB_RESCALE = 9  # drop the last limb, rounding (add_rns, no params)
 JOB_RESCALE_NTT = 10  # the same on NTT-form limbs, result in NTT form
+JOB_ADD_REF = 11  # elemfma(a, b, 1, q) per limb: the reference's vector add
 
 
 def _L():
@@ -386,7 +387,10 @@ class ServerConfig(ctypes.Structure):
                 ("max_inflight", ctypes.c_uint32),
                 ("max_batch", ctypes.c_uint32),
                 ("streams", ctypes.c_uint32),
-                ("queue_capacity", ctypes.c_uint32)]
+                ("queue_capacity", ctypes.c_uint32),
+                ("scratch_bytes", ctypes.c_uint64),
+                ("task_limbs", ctypes.c_uint32),
+                ("per_call", ctypes.c_uint32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -405,7 +409,8 @@ class ServerStats(ctypes.Structure):
                 ("batches", ctypes.c_uint64), ("transform_coeffs", ctypes.c_uint64),
                 ("inflight_bytes", ctypes.c_uint64),
                 ("peak_inflight_bytes", ctypes.c_uint64),
-                ("inflight", ctypes.c_uint32), ("waiting", ctypes.c_uint32)]
+                ("inflight", ctypes.c_uint32), ("waiting", ctypes.c_uint32),
+                ("reserved_bytes", ctypes.c_uint64)]
 
 
 OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_BUDGET, ERR_CANCELLED, ERR_TIMEOUT, \
@@ -423,10 +428,12 @@ class Server:
     """vkhel_server_*: continuous batching over device-resident operands."""
 
     def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=0, max_batch=0,
-                 streams=0, queue_capacity=0):
+                 streams=0, queue_capacity=0, scratch_bytes=0, task_limbs=0,
+                 per_call=False):
         self.ctx = ctx
         self.config = ServerConfig(hbm_budget_bytes, max_inflight, max_batch,
-                                   streams, queue_capacity)
+                                   streams, queue_capacity, scratch_bytes,
+                                   task_limbs, int(bool(per_call)))
         self._h = _L().vkhel_server_create(ctx.handle, ctypes.byref(self.config))
         if not self._h:
             _lib.check()
