@@ -136,6 +136,12 @@ int vkhel_ext_set_tile_bits(struct vkhel_ctx *, uint32_t logn, int tile_bits);
  * per tile (split).  The autotuner sets it too. */
 int vkhel_ext_pass_mode(struct vkhel_ctx *, uint32_t logn);
 int vkhel_ext_set_pass_mode(struct vkhel_ctx *, uint32_t logn, int mode);
+/* Pass mode 3 runs both passes of a two-pass size in one launch (batches
+ * and serving only; single calls keep mode 2): each XCD serves a queue of
+ * tiles in which the first pass of a polynomial runs `polys` polynomials
+ * ahead of its second, and hands the polynomial from one pass to the other
+ * through its own L2.  0 (default): derived from the occupancy. */
+int vkhel_ext_set_handoff_ahead(struct vkhel_ctx *, int polys);
 /* Tuned choices as text lines "log2n tile_bits pass_mode". */
 int vkhel_ext_autotune_save(struct vkhel_ctx *, const char *path);
 int vkhel_ext_autotune_load(struct vkhel_ctx *, const char *path);

@@ -108,6 +108,75 @@ MODS = [3, 5, 769, 65537, 1125891450734593, 1152921504606830593,
         (1 << 61) - 1, (1 << 63) + 29]
 
 
+def test_elemfma_wrap_branches_match_case_analysis(ctx):
+    """elemfma's `q - x` corrections (elemfma.comp:47-48, 51-52) on the GPU
+    against the case analysis of tests/mathref.py, every case reached."""
+    rng = np.random.default_rng(21)
+    seen = [0, 0, 0, 0]
+    for q in [3, 97, 1125891450734593, 1152921504606830593,
+              O.primes_ntt(1 << 61, 2, 1)[0]]:
+        count = 4000
+        a = np.concatenate([
+            rng.integers(0, U64MAX, count, dtype=np.uint64, endpoint=True),
+            np.uint64(U64MAX) - rng.integers(0, q, count, dtype=np.uint64),
+            rng.integers(0, q, count, dtype=np.uint64)])
+        b = rng.integers(0, q, a.size, dtype=np.uint64)
+        va, vb, out = vec(ctx, a), vec(ctx, b), ctx.vector(a.size)
+        for m in [1, 2, int(rng.integers(1, q)), q - 1]:
+            want, cases = R.elemfma_closed_form(a.tolist(), b.tolist(), m, q)
+            seen = [s + c for s, c in zip(seen, cases)]
+            va.elemfma(vb, out, m, q)
+            assert out.read().tolist() == want, (q, m)
+        for x in (va, vb, out):
+            x.close()
+    assert all(seen), seen
+
+
+@pytest.mark.parametrize("offset", [1, 3])
+def test_elementwise_on_odd_offset_views(ctx, offset):
+    """Views at odd word offsets are 8-byte aligned only; the pointwise
+    kernels then move one word per access.  Single calls and batch jobs on
+    such views against the oracle."""
+    q = 1152921504606830593
+    rng = np.random.default_rng(offset)
+    length = 10007
+    base_a = vec(ctx, rng.integers(0, U64MAX, length + 8, dtype=np.uint64,
+                                   endpoint=True))
+    base_b = vec(ctx, rng.integers(0, q, length + 8, dtype=np.uint64))
+    base_r = ctx.vector(length + 8)
+    a, b = base_a.view(offset, length), base_b.view(offset + 1, length)
+    r = base_r.view(offset + 2, length)
+    xa, xb = a.read(), b.read()
+    a.elemmul(b, r, q)
+    assert np.array_equal(r.read(), O.elemmul(xa, xb, q))
+    a.elemfma(b, r, 3, q)
+    assert np.array_equal(r.read(), O.elemfma(xa, xb, 3, q))
+    a.elemgtadd(r, q // 2, 12345)
+    assert np.array_equal(r.read(), O.elemgtadd(xa, q // 2, 12345))
+    a.elemgtsub(r, q // 2, 12345, 65537)
+    assert np.array_equal(r.read(), O.elemgtsub(xa, q // 2, 12345, 65537))
+    a.elemmod(r, 2, q)
+    assert np.array_equal(r.read(), O.elemmod(xa, 2, q))
+    # batch pointwise jobs on the same views (tables only name n and q)
+    n = 1 << 13
+    t = V.NttTables(n, q, O.root_2n(q, n))
+    ca = base_a.view(offset, n)
+    cb, cr = base_b.view(offset + 1, n), base_r.view(offset + 2, n)
+    ya, yb = ca.read() % np.uint64(q), cb.read()
+    ca.write(ya)
+    for op, want in [(V.JOB_MUL, O.elemmul(ya, yb, q)),
+                     (V.JOB_ADD, O.addmul_mod(ya, yb, 1, q)),
+                     (V.JOB_ADD_REF, O.elemfma(ya, yb, 1, q))]:
+        bt = V.Batch(ctx)
+        bt.add(op, ca, cb, cr, [t])
+        bt.run()
+        bt.close()
+        assert np.array_equal(cr.read(), want), op
+    for v in (a, b, r, ca, cb, cr, base_a, base_b, base_r):
+        v.close()
+    t.close()
+
+
 @pytest.mark.parametrize("length", [1, 2, 7, 64, 1000, 100003])
 def test_elementwise_vs_oracle(ctx, length):
     rng = np.random.default_rng(length)
@@ -240,6 +309,37 @@ def test_ntt_noncanonical_inputs_match_reference(ctx, logn, bits):
     t.close()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("algo", [V.NTT_FUSED, V.NTT_RADIX2])
+@pytest.mark.parametrize("bits", [50, 60, 61])
+@pytest.mark.parametrize("logn", [1, 2, 3, 4, 5, 6])
+def test_ntt_first_stage_wrap_matches_closed_form(ctx, logn, bits, algo):
+    """Words near 2^64 whose first-stage sum X + WY wraps (nttfwdbutterfly
+    .comp:56) against the closed form of tests/mathref.py (the DFT of the
+    corrected canonical vector), with both wrapped and unwrapped sums."""
+    n = 1 << logn
+    q = O.primes_ntt(1 << bits, 2 * n, 1)[0]
+    psi = O.root_2n(q, n)
+    t = V.NttTables(n, q, psi)
+    ctx.set_ntt_algo(algo)
+    rng = np.random.default_rng(logn * 100 + bits + 7)
+    wraps = 0
+    try:
+        for trial in range(8):
+            near = np.uint64(U64MAX) - rng.integers(0, q, n, dtype=np.uint64)
+            raw = rng.integers(0, U64MAX, n, dtype=np.uint64, endpoint=True)
+            x = np.where(rng.random(n) < 0.6, near, raw)
+            want, w = R.ntt_forward_noncanonical(x.tolist(), q, psi)
+            wraps += w
+            a = vec(ctx, x)
+            a.forward_transform(a, t)
+            assert a.read().tolist() == want, trial
+            a.close()
+    finally:
+        ctx.set_ntt_algo(V.NTT_AUTO)
+        t.close()
+    assert wraps > 0
 
 
 @pytest.mark.parametrize("logn", [2, 3, 5, 7])
@@ -1161,3 +1261,58 @@ def test_batch_with_70000_limbs(ctx):
         a.close()
         r.close()
         t.close()
+
+
+@pytest.mark.parametrize("tb", [11, 12, 13])
+@pytest.mark.parametrize("logn", [14, 16, 17])
+def test_one_launch_two_pass_matches_split(ctx, logn, tb):
+    """Pass mode 3 (both passes in one launch, k_ntt_handoff: per-XCD ticket
+    queues, producer tiles handing each polynomial to its consumer tiles
+    through the XCD's L2) gives the split kernels' words, forward and
+    inverse, over several primes and polynomial counts (including fewer
+    polynomials than XCDs); the batch's completion check passes."""
+    n = 1 << logn
+    qs = O.primes_ntt(1 << 50, 2 * n, 3) + O.primes_ntt(1 << 60, 2 * n, 1)
+    tabs = [V.NttTables(n, q, O.root_2n(q, n)) for q in qs]
+    try:
+        for npoly in (3, 37):
+            print(f"handoff n=2^{logn} tile=2^{tb} polys={npoly}", flush=True)
+            polys = []
+            for i in range(npoly):
+                v = ctx.vector(n, False)
+                v.fill_uniform(qs[i % len(qs)], 77 * logn + i)
+                polys.append(v)
+            want_f, want_i = [], []
+            ctx.set_tile_bits(logn, tb)
+            for mode in (2, 3):
+                ctx.set_pass_mode(logn, mode)
+                outs_f = [ctx.vector(n) for _ in polys]
+                outs_i = [ctx.vector(n) for _ in polys]
+                bt = V.Batch(ctx)
+                for i, (v, o) in enumerate(zip(polys, outs_f)):
+                    bt.add(V.JOB_NTT_FWD, v, None, o, [tabs[i % len(tabs)]])
+                bt.run()
+                bi = V.Batch(ctx)
+                for i, (o, r) in enumerate(zip(outs_f, outs_i)):
+                    bi.add(V.JOB_NTT_INV, o, None, r, [tabs[i % len(tabs)]])
+                bi.run()
+                got_f = [o.read() for o in outs_f]
+                got_i = [r.read() for r in outs_i]
+                if mode == 2:
+                    want_f, want_i = got_f, got_i
+                else:
+                    for i in range(npoly):
+                        assert np.array_equal(got_f[i], want_f[i]), ("fwd", i)
+                        assert np.array_equal(got_i[i], want_i[i]), ("inv", i)
+                        assert np.array_equal(got_i[i], polys[i].read()), ("roundtrip", i)
+                bt.close()
+                bi.close()
+                for v in outs_f + outs_i:
+                    v.close()
+            for v in polys:
+                v.close()
+    finally:
+        ctx.set_pass_mode(logn, 0)
+        ctx.set_tile_bits(logn, 0)
+        for t in tabs:
+            t.close()

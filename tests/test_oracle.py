@@ -246,21 +246,81 @@ def test_elementwise_closed_forms():
         assert O.elemmulconst(raw, c, q).tolist() == [int(x) * c % q for x in raw]
 
 
-def test_elemfma_matches_reference_including_wrap():
+def elemfma_inputs(rng, q, count=3000):
+    """Raw 64-bit a (half of them near 2^64) and canonical b."""
+    a = np.concatenate([rng.integers(0, 1 << 64, count, dtype=np.uint64),
+                        np.uint64((1 << 64) - 1) - rng.integers(0, q, count,
+                                                                 dtype=np.uint64),
+                        rng.integers(0, q, count, dtype=np.uint64)])
+    b = rng.integers(0, q, a.size, dtype=np.uint64)
+    return a, b
+
+
+def test_elemfma_wrap_branches_pinned_by_case_analysis():
+    """elemfma's two `q - x` corrections (elemfma.comp:47-48, 51-52) against
+    the case analysis of tests/mathref.py (derived from the Shoup quotient's
+    error, not from the shader's statements); every case is reached."""
     rng = np.random.default_rng(2)
-    for q in [769, 1125891450734593, 1152921504606830593]:
-        a = rng.integers(0, q, 1000, dtype=np.uint64)
-        b = rng.integers(0, q, 1000, dtype=np.uint64)
-        for m in [1, 2, 100, int(rng.integers(0, q))]:
-            got = O.elemfma(a, b, m, q).tolist()
-            assert got == R.elemfma_reference_wrap(a.tolist(), b.tolist(), m, q)
-            correct = [(int(x) * m + int(y)) % q for x, y in zip(a, b)]
-            # agrees with the true FMA exactly where no wrap happens
-            for g, c, x, y in zip(got, correct, a, b):
-                if (int(x) * m % q) + int(y) < q:
-                    assert g == c
+    seen = [0, 0, 0, 0]
+    for q in [3, 97, 769, 1125891450734593, 1152921504606830593,
+              2305843009213693951 - 30]:
+        if not O.is_prime(q):
+            q = O.primes_ntt(q + 1, 2, 1)[0]
+        a, b = elemfma_inputs(rng, q)
+        for m in [1, 2, 100 % q or 1, int(rng.integers(1, q)), q - 1]:
+            want, cases = R.elemfma_closed_form(a.tolist(), b.tolist(), m, q)
+            assert O.elemfma(a, b, m, q).tolist() == want, (q, m)
+            seen = [s + c for s, c in zip(seen, cases)]
+            # the true (a m + b) mod q exactly in the first case
+            if cases[0]:
+                ok = [(int(x) * m % q) + int(y) < q and
+                      ((int(x) * m // q) == ((int(x) * ((m << 64) // q)) >> 64))
+                      for x, y in zip(a, b)]
+                got = O.elemfma(a, b, m, q)
+                for g, x, y, k in zip(got.tolist(), a.tolist(), b.tolist(), ok):
+                    if k:
+                        assert g == (x * m + y) % q
         assert O.addmul_mod(a, b, 5, q).tolist() == [
             (int(x) * 5 + int(y)) % q for x, y in zip(a, b)]
+    assert all(seen), seen
+
+
+def test_elemfma_closed_form_rejects_planted_mistakes():
+    """A restatement that kept `p - q` (the intended correction) or dropped
+    the sum's correction would differ from the reference's words."""
+    rng = np.random.default_rng(3)
+    q = 1152921504606830593
+    a, b = elemfma_inputs(rng, q, 500)
+    want, _ = R.elemfma_closed_form(a.tolist(), b.tolist(), 1, q)
+    intended = [(int(x) + int(y)) % q for x, y in zip(a, b)]
+    assert want != intended
+    unwrapped = [((int(x) % q) + int(y)) for x, y in zip(a, b)]
+    assert want != unwrapped
+
+
+@pytest.mark.parametrize("bits", [50, 60, 61])
+@pytest.mark.parametrize("logn", [1, 2, 3, 4, 5, 6])
+def test_forward_first_stage_wrap_pinned_by_closed_form(logn, bits):
+    """The forward transform on words near 2^64, where the first stage's
+    64-bit X + WY wraps (nttfwdbutterfly.comp:56), against the closed-form
+    DFT of the corrected canonical vector (tests/mathref.py); wrapped and
+    unwrapped sums both occur."""
+    n = 1 << logn
+    q = O.primes_ntt(1 << bits, 2 * n, 1)[0]
+    psi = O.root_2n(q, n)
+    rng = np.random.default_rng(logn * 100 + bits)
+    wraps = 0
+    for trial in range(12):
+        near = np.uint64((1 << 64) - 1) - rng.integers(0, q, n, dtype=np.uint64)
+        raw = rng.integers(0, 1 << 64, n, dtype=np.uint64)
+        x = np.where(rng.random(n) < 0.6, near, raw)
+        want, w = R.ntt_forward_noncanonical(x.tolist(), q, psi)
+        wraps += w
+        assert O.ntt_forward(x, q, psi).tolist() == want
+        # a restatement without the wrap correction would differ
+        if w:
+            assert R.dft_negacyclic_bitrev([int(v) % q for v in x], q, psi) != want
+    assert 0 < wraps < 12 * n // 2
 
 
 def test_fill_uniform_is_seeded_and_in_range():

@@ -59,9 +59,16 @@ int vkhel_ext_pass_mode(struct vkhel_ctx *ctx, uint32_t logn) {
 }
 
 int vkhel_ext_set_pass_mode(struct vkhel_ctx *ctx, uint32_t logn, int mode) {
-	if (!ctx || logn >= 64 || mode < 0 || mode > 2)
-		return fail(VKHEL_ERR_INVALID, "set_pass_mode: mode must be 0, 1 or 2");
+	if (!ctx || logn >= 64 || mode < 0 || mode > 3)
+		return fail(VKHEL_ERR_INVALID, "set_pass_mode: mode must be 0..3");
 	ctx->pass_mode[logn] = mode;
+	return VKHEL_OK;
+}
+
+int vkhel_ext_set_handoff_ahead(struct vkhel_ctx *ctx, int polys) {
+	if (!ctx || polys < 0 || polys > 4096)
+		return fail(VKHEL_ERR_INVALID, "set_handoff_ahead: 0..4096 polynomials");
+	ctx->handoff_ahead = polys;
 	return VKHEL_OK;
 }
 
@@ -212,7 +219,7 @@ int vkhel_ext_autotune_load(struct vkhel_ctx *ctx, const char *path) {
 	int rc = VKHEL_OK;
 	while (fscanf(f, "%u %d %d", &l, &tb, &mode) == 3) {
 		if (l >= ctx->tile_bits.size() || (tb != 0 && (tb < 11 || tb > 13)) ||
-				mode < 0 || mode > 2) {
+				mode < 0 || mode > 3) {
 			rc = fail(VKHEL_ERR_INVALID, "autotune_load: bad entry %u %d %d", l, tb,
 					mode);
 			break;

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "handoff.h"
 #include "jobs.h"
 #include "modarith.h"
 #include "numbers.h"
@@ -64,13 +65,22 @@ __device__ __forceinline__ uint64_t point_op(const PointTask &t, uint64_t x,
 	}
 }
 
-// One workgroup row per task (blockIdx.y); unary ops never read b.
+// One workgroup row per task (blockIdx.y); unary ops never read b.  Two
+// words per access when the task's pointers are 16-byte aligned, one
+// otherwise (vector views at odd word offsets).
 __global__ __launch_bounds__(256) void k_pointwise_tasks(const PointTask *tasks) {
 	const PointTask t = tasks[blockIdx.y];
 	const bool binary = t.op == kPtMul || t.op == kPtAdd || t.op == kPtRescale ||
 		t.op == kPtFma;
-	const uint64_t pairs = t.len >> 1;
 	const uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
+	if ((((uintptr_t) t.a | (uintptr_t) t.out |
+					(binary ? (uintptr_t) t.b : 0)) & 15) != 0) {
+		for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+				i < t.len; i += stride)
+			t.out[i] = point_op(t, t.a[i], binary ? t.b[i] : 0);
+		return;
+	}
+	const uint64_t pairs = t.len >> 1;
 	for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
 			i < pairs; i += stride) {
 		const ulonglong2 x = ((const ulonglong2 *) t.a)[i];
@@ -96,6 +106,9 @@ void DeviceTasks::release() {
 	(void) hipFree(pt);
 	(void) hipFree(post);
 	(void) hipFree(post2);
+	(void) hipFree(handoff);
+	handoff = nullptr;
+	handoff_bytes = 0;
 	fwd = inv = pm_col = pm = fwd2 = nullptr;
 	pt = post = post2 = nullptr;
 	cap_fwd = cap_inv = cap_pm_col = cap_pm = cap_fwd2 = 0;
@@ -115,15 +128,29 @@ static bool reserve_array(T **d, size_t *cap, size_t n) {
 	return true;
 }
 
+static bool reserve_handoff(DeviceTasks &dt, uint64_t bytes) {
+	if (dt.handoff_bytes >= bytes)
+		return true;
+	(void) hipFree(dt.handoff);
+	dt.handoff = nullptr;
+	dt.handoff_bytes = 0;
+	if (!hip_ok(hipMalloc(&dt.handoff, bytes), "hipMalloc(handoff state)"))
+		return false;
+	dt.handoff_bytes = bytes;
+	return true;
+}
+
 bool DeviceTasks::reserve(size_t n) {
 	return reserve_array(&fwd, &cap_fwd, n) && reserve_array(&inv, &cap_inv, n) &&
 		reserve_array(&pm_col, &cap_pm_col, n) && reserve_array(&pm, &cap_pm, n) &&
 		reserve_array(&fwd2, &cap_fwd2, n) && reserve_array(&pt, &cap_pt, n) &&
-		reserve_array(&post, &cap_post, n) && reserve_array(&post2, &cap_post2, n);
+		reserve_array(&post, &cap_post, n) && reserve_array(&post2, &cap_post2, n) &&
+		reserve_handoff(*this, handoff_state_bytes((uint32_t) n));
 }
 
 uint64_t DeviceTasks::bytes_for(size_t n) {
-	return (uint64_t) n * (5 * sizeof(NttTask) + 3 * sizeof(PointTask));
+	return (uint64_t) n * (5 * sizeof(NttTask) + 3 * sizeof(PointTask)) +
+		handoff_state_bytes((uint32_t) n);
 }
 
 int make_job(const vkhel_ctx *ctx, int op, const vkhel_vector *a,
@@ -254,7 +281,7 @@ static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
 		const int mc = modulus_class(tasks[i].c.q);
 		if (segs.empty() || segs.back().logn != tasks[i].c.logn ||
 				segs.back().mc != mc)
-			segs.push_back(Segment{i, 0, tasks[i].c.logn, mc, 0});
+			segs.push_back(Segment{i, 0, tasks[i].c.logn, mc, 0, kNoHandoff});
 		segs.back().count++;
 	}
 	return segs;
@@ -475,6 +502,44 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 		for (Segment &g : *segs)
 			g.tb = polymul_fused_tile(ctx, g.logn);
 	}
+	// segments that run both passes in one launch get state words of
+	// their own, so every one can be checked after the launch
+	for (auto *segs : {&ts.seg_fwd, &ts.seg_inv, &ts.seg_fwd2}) {
+		for (Segment &g : *segs) {
+			if (uses_handoff(ctx, g.logn)) {
+				g.tb = tile_bits_for(ctx, g.logn);
+				g.hoff = (uint32_t) ts.handoff_words;
+				ts.handoff_words += handoff_state_bytes(g.count) / 4;
+			}
+		}
+	}
+	return VKHEL_OK;
+}
+
+int check_handoff(const TaskSet &ts, const DeviceTasks &dt) {
+	if (!ts.handoff_words)
+		return VKHEL_OK;
+	std::vector<uint32_t> h(ts.handoff_words);
+	if (!hip_ok(hipMemcpy(h.data(), dt.handoff, 4 * h.size(),
+					hipMemcpyDeviceToHost), "read handoff state"))
+		return vkhel_ext_last_error();
+	for (const auto *segs : {&ts.seg_fwd, &ts.seg_inv, &ts.seg_fwd2}) {
+		for (const Segment &g : *segs) {
+			if (g.hoff == kNoHandoff)
+				continue;
+			const uint32_t *st = h.data() + g.hoff;
+			const uint32_t tiles = handoff_tiles(g.logn, g.tb);
+			bool ok = st[handoff_err_word()] == 0;
+			for (uint32_t x = 0; x < kXcds; x++)
+				ok = ok && st[handoff_done_word(x)] ==
+					handoff_queue_tickets(g.count, x, tiles);
+			if (!ok)
+				return fail(VKHEL_ERR_DEVICE, "one-launch two-pass transform of %u "
+						"polynomials (n = 2^%u) did not complete its schedule "
+						"(error flag %u); its results are invalid", g.count, g.logn,
+						st[handoff_err_word()]);
+		}
+	}
 	return VKHEL_OK;
 }
 
@@ -497,6 +562,8 @@ static bool upload_array(const std::vector<T> &h, T **d, size_t *cap,
 }
 
 int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s) {
+	if (ts.handoff_words && !reserve_handoff(dt, 4 * ts.handoff_words))
+		return vkhel_ext_last_error();
 	if (!upload_array(ts.fwd, &dt.fwd, &dt.cap_fwd, s) ||
 			!upload_array(ts.inv, &dt.inv, &dt.cap_inv, s) ||
 			!upload_array(ts.pm_col, &dt.pm_col, &dt.cap_pm_col, s) ||
@@ -511,12 +578,14 @@ int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s) {
 
 static int launch_segments(const vkhel_ctx *ctx,
 		const std::vector<Segment> &segs, const std::vector<NttTask> &host,
-		const NttTask *dev, bool inverse, hipStream_t s) {
+		const NttTask *dev, bool inverse, const DeviceTasks &dt, hipStream_t s) {
 	for (const Segment &g : segs) {
 		int rc = VKHEL_OK;
 		if (ntt_fused_supported(g.logn)) {
-			rc = launch_fused_tasks(ctx, dev + g.off, g.count, g.logn, inverse,
-					g.mc, s);
+			rc = g.hoff == kNoHandoff
+				? launch_fused_tasks(ctx, dev + g.off, g.count, g.logn, inverse, g.mc, s)
+				: launch_fused_tasks(ctx, dev + g.off, g.count, g.logn, inverse, g.mc, s,
+						dt.handoff + g.hoff, dt.handoff_bytes - 4ull * g.hoff);
 		} else {
 			for (uint32_t i = 0; i < g.count && rc == VKHEL_OK; i++)
 				rc = launch_task_ntt(host[g.off + i], inverse, s);
@@ -568,17 +637,17 @@ static int launch_polymul_segments(const TaskSet &ts, const DeviceTasks &dt,
 
 int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
 		const DeviceTasks &dt, hipStream_t s) {
-	int rc = launch_segments(ctx, ts.seg_fwd, ts.fwd, dt.fwd, false, s);
+	int rc = launch_segments(ctx, ts.seg_fwd, ts.fwd, dt.fwd, false, dt, s);
 	if (rc == VKHEL_OK)
 		rc = launch_polymul_segments(ts, dt, s);
 	if (rc == VKHEL_OK)
 		rc = launch_pointwise(ts.pt, dt.pt, ts.pt_maxlen, s);
 	if (rc == VKHEL_OK)
-		rc = launch_segments(ctx, ts.seg_inv, ts.inv, dt.inv, true, s);
+		rc = launch_segments(ctx, ts.seg_inv, ts.inv, dt.inv, true, dt, s);
 	if (rc == VKHEL_OK)
 		rc = launch_pointwise(ts.post, dt.post, ts.post_maxlen, s);
 	if (rc == VKHEL_OK)
-		rc = launch_segments(ctx, ts.seg_fwd2, ts.fwd2, dt.fwd2, false, s);
+		rc = launch_segments(ctx, ts.seg_fwd2, ts.fwd2, dt.fwd2, false, dt, s);
 	if (rc == VKHEL_OK)
 		rc = launch_pointwise(ts.post2, dt.post2, ts.post2_maxlen, s);
 	return rc;
@@ -638,7 +707,7 @@ static int wait_launched(vkhel_batch *b) {
 	if (!hip_ok(hipEventSynchronize(b->done), "batch wait"))
 		return vkhel_ext_last_error();
 	b->launched = false;
-	return VKHEL_OK;
+	return check_handoff(b->ts, b->dt);
 }
 
 // Turns the job list into device task arrays and a captured graph.

@@ -65,33 +65,47 @@ struct OpModTwo {
 	}
 };
 
-template <class Op>
+// VEC: every pointer is 16-byte aligned and two words move per access;
+// otherwise (a vector view at an odd word offset) one word per access.
+template <class Op, bool VEC>
 __global__ __launch_bounds__(256) void k_binary(const uint64_t *a,
 		const uint64_t *b, uint64_t *out, uint64_t len, Op op) {
-	const uint64_t pairs = len >> 1;
 	const uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
-	for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-			i < pairs; i += stride) {
-		const ulonglong2 x = ((const ulonglong2 *) a)[i];
-		const ulonglong2 y = ((const ulonglong2 *) b)[i];
-		((ulonglong2 *) out)[i] = make_ulonglong2(op(x.x, y.x), op(x.y, y.y));
+	uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if constexpr (VEC) {
+		for (const uint64_t pairs = len >> 1; i < pairs; i += stride) {
+			const ulonglong2 x = ((const ulonglong2 *) a)[i];
+			const ulonglong2 y = ((const ulonglong2 *) b)[i];
+			((ulonglong2 *) out)[i] = make_ulonglong2(op(x.x, y.x), op(x.y, y.y));
+		}
+		if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+			out[len - 1] = op(a[len - 1], b[len - 1]);
+	} else {
+		for (; i < len; i += stride)
+			out[i] = op(a[i], b[i]);
 	}
-	if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0)
-		out[len - 1] = op(a[len - 1], b[len - 1]);
 }
 
-template <class Op>
+template <class Op, bool VEC>
 __global__ __launch_bounds__(256) void k_unary(const uint64_t *a,
 		uint64_t *out, uint64_t len, Op op) {
-	const uint64_t pairs = len >> 1;
 	const uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
-	for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-			i < pairs; i += stride) {
-		const ulonglong2 x = ((const ulonglong2 *) a)[i];
-		((ulonglong2 *) out)[i] = make_ulonglong2(op(x.x), op(x.y));
+	uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if constexpr (VEC) {
+		for (const uint64_t pairs = len >> 1; i < pairs; i += stride) {
+			const ulonglong2 x = ((const ulonglong2 *) a)[i];
+			((ulonglong2 *) out)[i] = make_ulonglong2(op(x.x), op(x.y));
+		}
+		if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+			out[len - 1] = op(a[len - 1]);
+	} else {
+		for (; i < len; i += stride)
+			out[i] = op(a[i]);
 	}
-	if ((len & 1) && blockIdx.x == 0 && threadIdx.x == 0)
-		out[len - 1] = op(a[len - 1]);
+}
+
+static bool aligned16(const void *p) {
+	return ((uintptr_t) p & 15) == 0;
 }
 
 // Synthetic residues: high half of splitmix64(seed, i) * q.
@@ -117,8 +131,12 @@ int launch_binary(const uint64_t *a, const uint64_t *b, uint64_t *out,
 		uint64_t len, Op op, hipStream_t s) {
 	if (len == 0)
 		return VKHEL_OK;
-	hipLaunchKernelGGL(k_binary<Op>, dim3(grid_for(len)), dim3(256), 0, s, a, b,
-			out, len, op);
+	if (aligned16(a) && aligned16(b) && aligned16(out))
+		hipLaunchKernelGGL((k_binary<Op, true>), dim3(grid_for(len)), dim3(256), 0,
+				s, a, b, out, len, op);
+	else
+		hipLaunchKernelGGL((k_binary<Op, false>), dim3(grid_for(2 * len)),
+				dim3(256), 0, s, a, b, out, len, op);
 	return hip_ok(hipGetLastError(), "launch") ? VKHEL_OK : VKHEL_ERR_DEVICE;
 }
 
@@ -127,8 +145,12 @@ int launch_unary(const uint64_t *a, uint64_t *out, uint64_t len, Op op,
 		hipStream_t s) {
 	if (len == 0)
 		return VKHEL_OK;
-	hipLaunchKernelGGL(k_unary<Op>, dim3(grid_for(len)), dim3(256), 0, s, a, out,
-			len, op);
+	if (aligned16(a) && aligned16(out))
+		hipLaunchKernelGGL((k_unary<Op, true>), dim3(grid_for(len)), dim3(256), 0,
+				s, a, out, len, op);
+	else
+		hipLaunchKernelGGL((k_unary<Op, false>), dim3(grid_for(2 * len)),
+				dim3(256), 0, s, a, out, len, op);
 	return hip_ok(hipGetLastError(), "launch") ? VKHEL_OK : VKHEL_ERR_DEVICE;
 }
 

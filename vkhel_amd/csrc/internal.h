@@ -96,8 +96,13 @@ struct vkhel_ctx {
 	// Tuned tile size (log2 words) per log2 n; 0 = default.
 	std::vector<int> tile_bits = std::vector<int>(64, 0);
 	// Tuned pass mode per log2 n: 0 = by batch size, 1 = whole polynomial
-	// per workgroup, 2 = one grid per pass (one workgroup per tile).
+	// per workgroup, 2 = one grid per pass (one workgroup per tile), 3 = both
+	// passes in one persistent launch (batches only).
 	std::vector<int> pass_mode = std::vector<int>(64, 0);
+	// Look-ahead of the one-launch two-pass schedule in polynomials
+	// (handoff.h; 0: from the occupancy).
+	int handoff_ahead = 0;
+
 	std::mutex mu;  // guards `tables` (batch and serve threads share it)
 	std::unordered_map<uint64_t, vk::DevTables *> tables;
 };
@@ -183,8 +188,15 @@ NttTask make_task(const DevTables &t, const uint64_t *in, uint64_t *out,
 // Device-resident tasks that share log2 n (which selects the compiled
 // kernel), direction and modulus class (mc, modulus_class() of every q);
 // the context supplies the tuned tile size.
+// Pass mode 3 (one launch for both passes, k_ntt_handoff) needs a state
+// buffer of handoff_state_bytes(ntasks); without one the passes run split.
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
-		uint32_t ntasks, uint32_t logn, bool inverse, int mc, hipStream_t st);
+		uint32_t ntasks, uint32_t logn, bool inverse, int mc, hipStream_t st,
+		uint32_t *handoff_state = nullptr, uint64_t handoff_bytes = 0);
+uint64_t handoff_state_bytes(uint32_t ntasks);
+// True when transforms of log2 n run both passes in one launch on ctx
+// (pass mode 3 at a two-pass size).
+bool uses_handoff(const vkhel_ctx *ctx, uint32_t logn);
 // Fused polynomial products (k_ntt_pm): the tile size (log2 words) they
 // run on for log2 n on this context, 0 when that size and tile do not allow
 // them.

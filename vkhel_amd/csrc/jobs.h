@@ -48,7 +48,11 @@ struct Segment {
 	uint32_t off, count, logn;
 	int mc;
 	int tb;
+	// word offset of this segment's state in DeviceTasks::handoff when it
+	// runs both passes in one launch (pass mode 3), else kNoHandoff
+	uint32_t hoff;
 };
+constexpr uint32_t kNoHandoff = 0xffffffffu;
 
 struct TaskSet {
 	std::vector<NttTask> fwd, inv;
@@ -65,6 +69,7 @@ struct TaskSet {
 	std::vector<Segment> seg_fwd2;
 	std::vector<PointTask> post2;
 	uint64_t post2_maxlen = 0;
+	uint64_t handoff_words = 0;  // state words of all one-launch segments
 };
 
 // Device copies of a TaskSet's arrays, grown on demand.
@@ -74,6 +79,9 @@ struct DeviceTasks {
 	PointTask *pt = nullptr, *post = nullptr, *post2 = nullptr;
 	size_t cap_fwd = 0, cap_inv = 0, cap_pm_col = 0, cap_pm = 0, cap_fwd2 = 0;
 	size_t cap_pt = 0, cap_post = 0, cap_post2 = 0;
+	// state of one-launch two-pass transforms (pass mode 3)
+	uint32_t *handoff = nullptr;
+	uint64_t handoff_bytes = 0;
 	void release();
 	// Allocates every array for `tasks` entries up front (false with the
 	// error recorded on failure).
@@ -105,6 +113,9 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 // Copies the task arrays to `dt` (stream-ordered) and enqueues the
 // phases on `s`.
 int upload_tasks(const TaskSet &ts, DeviceTasks &dt, hipStream_t s);
+// After the launch has completed: VKHEL_OK if every one-launch two-pass
+// segment ran all its tiles with no wait given up, else the error recorded.
+int check_handoff(const TaskSet &ts, const DeviceTasks &dt);
 int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
 		const DeviceTasks &dt, hipStream_t s);
 

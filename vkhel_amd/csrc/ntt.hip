@@ -21,6 +21,8 @@
 // Sizes above 2^20 and the RADIX2 algorithm use the radix-2 kernels below.
 #include <algorithm>
 #include <array>
+#include <mutex>
+#include <unordered_map>
 #include <utility>
 
 #include "internal.h"
@@ -235,20 +237,82 @@ static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
 // unless tuned otherwise: at every size and batch measured, the split grids'
 // smaller register footprint and full-chip occupancy beat one workgroup per
 // polynomial.
+static int pass_mode_of(const vkhel_ctx *ctx, uint32_t logn) {
+	return logn < ctx->pass_mode.size() ? ctx->pass_mode[logn] : 0;
+}
+
 static bool use_split(const vkhel_ctx *ctx, uint32_t logn) {
-	const int mode = logn < ctx->pass_mode.size() ? ctx->pass_mode[logn] : 0;
-	return mode != 1;
+	return pass_mode_of(ctx, logn) != 1;
+}
+
+// Zeroes the state words of a one-launch two-pass transform: a kernel, not
+// hipMemsetAsync, so that it is safe inside captured graphs.
+__global__ __launch_bounds__(256) void k_handoff_reset(uint32_t *p, uint32_t n) {
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+			i += gridDim.x * blockDim.x)
+		p[i] = 0;
+}
+
+// Polls a consumer tile makes before it gives up and sets the error flag
+// (each poll is an atomic round trip and a short sleep, about a microsecond;
+// a producer tile takes tens of microseconds at most).
+constexpr uint32_t kHandoffSpinLimit = 1u << 20;
+
+uint64_t handoff_state_bytes(uint32_t ntasks) {
+	return 4ull * handoff_state_words(ntasks);
+}
+
+bool uses_handoff(const vkhel_ctx *ctx, uint32_t logn) {
+	return fused_supported(logn) && pass_mode_of(ctx, logn) == 3 &&
+		fused_entry(logn, tile_bits_for(ctx, logn)).tiles > 1;
+}
+
+// Both passes of `ntasks` two-pass transforms in one persistent launch
+// (k_ntt_handoff), its state in `state` (handoff_state_bytes(ntasks)).
+static int launch_handoff(const vkhel_ctx *ctx, const FusedEntry &e,
+		const NttTask *dev_tasks, uint32_t ntasks, bool inverse, int mc,
+		uint32_t *state, hipStream_t st) {
+	const uint32_t words = handoff_state_words(ntasks);
+	hipLaunchKernelGGL(k_handoff_reset, dim3(std::min<uint32_t>((words + 255) / 256,
+					1024)), dim3(256), 0, st, state, words);
+	const HandoffKernel k = inverse ? e.inv_h[mc] : e.fwd_h[mc];
+	// one workgroup per ticket: placed round-robin over the XCDs, this grid
+	// gives every XCD as many workgroups as the longest queue (XCD 0's) has
+	// tickets
+	const uint64_t grid = (uint64_t) kXcds *
+		handoff_queue_tickets(ntasks, 0, (uint32_t) e.tiles);
+	if (grid > 0x7fffffffull)
+		return fail(VKHEL_ERR_INVALID, "ntt handoff: %u polynomials is too many "
+				"for one launch", ntasks);
+	// look-ahead: about the polynomials the resident workgroups of one XCD
+	// hold tickets of (four 512-thread workgroups per CU, 32 CUs per XCD)
+	// unless set on the context
+	const uint32_t ahead = ctx->handoff_ahead > 0 ? (uint32_t) ctx->handoff_ahead
+		: std::max<uint32_t>(1, (uint32_t) (ctx->cu_count / kXcds) *
+				std::max(1, 2048 / e.threads) / (2 * e.tiles) + 1);
+	const NttTask none{};
+	hipLaunchKernelGGL(k, dim3((uint32_t) grid), dim3(e.threads), 0, st, none,
+			dev_tasks, ntasks, state, kHandoffSpinLimit, ahead);
+	return hip_ok(hipGetLastError(), "ntt handoff launch") ? VKHEL_OK
+		: VKHEL_ERR_DEVICE;
 }
 
 int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
-		uint32_t ntasks, uint32_t logn, bool inverse, int mc, hipStream_t st) {
+		uint32_t ntasks, uint32_t logn, bool inverse, int mc, hipStream_t st,
+		uint32_t *handoff_state, uint64_t handoff_bytes) {
 	if (ntasks == 0)
 		return VKHEL_OK;
 	if (!fused_supported(logn))
 		return fail(VKHEL_ERR_INVALID, "fused NTT: unsupported log2 n = %u", logn);
+	const int tb = tile_bits_for(ctx, logn);
+	const FusedEntry &e = fused_entry(logn, tb);
+	if (pass_mode_of(ctx, logn) == 3 && e.tiles > 1 && dev_tasks &&
+			handoff_state && handoff_bytes >= handoff_state_bytes(ntasks))
+		return launch_handoff(ctx, e, dev_tasks, ntasks, inverse, mc,
+				handoff_state, st);
 	const NttTask none{};
-	return launch_fused_impl(none, dev_tasks, ntasks, logn,
-			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse, mc, st);
+	return launch_fused_impl(none, dev_tasks, ntasks, logn, tb,
+			use_split(ctx, logn), inverse, mc, st);
 }
 
 int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn) {

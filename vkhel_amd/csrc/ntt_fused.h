@@ -5,6 +5,7 @@
 #include <array>
 #include <utility>
 
+#include "handoff.h"
 #include "internal.h"
 #include "modarith.h"
 
@@ -13,6 +14,11 @@ namespace vk {
 typedef void (*FusedKernel)(NttTask, const NttTask *);
 
 typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
+
+// Both passes of a two-pass size in one persistent grid (k_ntt_handoff):
+// (uniform, tasks, polynomials, state words, spin limit).
+typedef void (*HandoffKernel)(NttTask, const NttTask *, uint32_t, uint32_t *,
+		uint32_t, uint32_t);
 
 // One compiled (log2 n, tile size) variant, per modulus class (index
 // kClassGeneral / kClassMid / kClassSmall, modulus_class()): whole-transform
@@ -26,6 +32,8 @@ struct FusedEntry {
 	// tile allow it)
 	PassKernel pm[3];
 	bool pm_ok;
+	// one-launch two-pass transforms (two-pass sizes only)
+	HandoffKernel fwd_h[3], inv_h[3];
 	int threads;
 	int tiles;  // tiles per pass (1: one-tile size, passes unused)
 };
@@ -884,6 +892,105 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 	}
 }
 
+// ---- both passes in one launch (handoff.h) ----
+//
+// The XCD (its own L2) this wave runs on.
+__device__ __forceinline__ uint32_t xcc_id() {
+	uint32_t v;
+	asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+	return v & (kXcds - 1);
+}
+
+// An atomic read of *p performed where the atomics are (a compare-and-swap
+// that never changes the word), not a load a cache above may satisfy.
+__device__ __forceinline__ uint32_t atomic_peek(uint32_t *p) {
+	uint32_t e = 0;
+	__hip_atomic_compare_exchange_strong(p, &e, 0u, __ATOMIC_RELAXED,
+			__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return e;
+}
+
+// One workgroup per ticket: each workgroup claims the next ticket of its
+// XCD's queue (handoff_decode) and runs that tile of the polynomial's first
+// or second pass; the grid gives every XCD as many workgroups as the
+// longest queue has tickets, and the ones left without a ticket exit.  (A
+// persistent loop over tickets needs about twice the registers.)  A
+// producer tile publishes by waiting for its stores to complete (they are
+// then in the XCD's L2: the vector L1 writes through) and incrementing the
+// polynomial's count.  A consumer tile first waits for the count to reach
+// the tile count, polling at most spin_limit times (then it sets the error
+// flag and goes on, and no later wait starts, so the launch always ends),
+// and invalidates its CU's vector L1 so that its loads see the L2.  Tasks
+// come from the device array (read-only: restrict lets them load through
+// the scalar cache).
+template <int LOGN, bool INV, int TB, int MC>
+__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
+__attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, 1, TB)))) void k_ntt_handoff(
+		NttTask, const NttTask *__restrict__ tasks, uint32_t npoly,
+		uint32_t *state, uint32_t spin_limit, uint32_t ahead) {
+	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
+	__shared__ uint32_t s_ticket;
+	if constexpr (LOGN > TB) {
+		using G1 = Geo<LOGN, 1, TB>;
+		using G2 = Geo<LOGN, 2, TB>;
+		constexpr uint32_t T = G1::NTILES;
+		const uint32_t t = threadIdx.x;
+		const uint32_t x = xcc_id();
+		if (t == 0)
+			s_ticket = __hip_atomic_fetch_add(state + x * kHandoffStride, 1u,
+					__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__syncthreads();
+		// (uniform, so the task loads through the scalar cache)
+		const uint32_t k = __builtin_amdgcn_readfirstlane(s_ticket);
+		if (k >= handoff_queue_tickets(npoly, x, T))
+			return;
+		const HandoffWork w = handoff_decode(k, x, npoly, T, ahead);
+		uint32_t *count = state + kHandoffCounts + x * handoff_count_stride(npoly) +
+			w.poly / kXcds;
+		const NttTask tk = tasks[w.poly];
+		if (!w.producer) {
+			// (once any wait has given up, none waits again)
+			if (t == 0 && !atomic_peek(state + kHandoffErr)) {
+				uint32_t spins = 0;
+				while (atomic_peek(count) < T) {
+					if (++spins >= spin_limit) {
+						__hip_atomic_fetch_or(state + kHandoffErr, 1u, __ATOMIC_RELAXED,
+								__HIP_MEMORY_SCOPE_AGENT);
+						break;
+					}
+					__builtin_amdgcn_s_sleep(2);
+				}
+			}
+			__syncthreads();
+			asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+		}
+		// forward: producer = column pass (1), consumer = row pass (2);
+		// inverse: producer = row pass (2), consumer = column pass (1)
+		if (w.producer == INV) {
+			const uint32_t jb = w.tile << TB;
+			run_rounds<G2, INV, LOGN, false, MC>(
+					std::make_integer_sequence<int, G2::ROUNDS>(), t,
+					w.producer ? tk.in : tk.out, tk.in2, tk.out, lds, jb, jb, tk.tw,
+					tk.c, !w.producer);
+		} else {
+			run_rounds<G1, INV, LOGN, false, MC>(
+					std::make_integer_sequence<int, G1::ROUNDS>(), t,
+					w.producer ? tk.in : tk.out, nullptr, tk.out, lds,
+					w.tile << G1::CB, 0, tk.tw, tk.c, !w.producer);
+		}
+		if (w.producer) {
+			asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+			__syncthreads();
+			if (t == 0)
+				__hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED,
+						__HIP_MEMORY_SCOPE_AGENT);
+		}
+		if (t == 0)
+			__hip_atomic_fetch_add(state + handoff_done_word(x), 1u,
+					__ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+}
+
 // The middle of a polynomial product a*b fused into one kernel: a's
 // forward rounds (twiddles tw2) with canonical results kept in registers,
 // the pointwise product with b's spectrum (in2) at the same words, then the
@@ -983,6 +1090,12 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 			k_ntt_pm<L + kMinFusedLogn, TB, 1>,
 			k_ntt_pm<L + kMinFusedLogn, TB, 2>},
 		pm_fusable<L + kMinFusedLogn, TB>(),
+		{k_ntt_handoff<L + kMinFusedLogn, false, TB, 0>,
+			k_ntt_handoff<L + kMinFusedLogn, false, TB, 1>,
+			k_ntt_handoff<L + kMinFusedLogn, false, TB, 2>},
+		{k_ntt_handoff<L + kMinFusedLogn, true, TB, 0>,
+			k_ntt_handoff<L + kMinFusedLogn, true, TB, 1>,
+			k_ntt_handoff<L + kMinFusedLogn, true, TB, 2>},
 		FusedShape<L + kMinFusedLogn, TB>::THREADS,
 		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0)}...};
 }

@@ -102,6 +102,8 @@ static int run_micro_batch(vkhel_server *s, Worker &w,
 		rc = enqueue_tasks(s->ctx, w.ts, w.dt, w.stream);
 	if (rc == VKHEL_OK && !hip_ok(hipStreamSynchronize(w.stream), "serve sync"))
 		rc = VKHEL_ERR_DEVICE;
+	if (rc == VKHEL_OK)
+		rc = check_handoff(w.ts, w.dt);
 	return rc;
 }
 

@@ -77,6 +77,11 @@ class Context:
     def set_pass_mode(self, logn, mode):
         _lib.check(_L().vkhel_ext_set_pass_mode(self._h, logn, mode))
 
+    def set_handoff_ahead(self, polys):
+        """vkhel_ext_set_handoff_ahead: look-ahead of pass mode 3's schedule
+        (0: from the occupancy)."""
+        _lib.check(_L().vkhel_ext_set_handoff_ahead(self._h, polys))
+
     def autotune_save(self, path):
         _lib.check(_L().vkhel_ext_autotune_save(self._h, str(path).encode()))
 
