@@ -31,6 +31,8 @@ Also reported, each with its own timing:
     arrival rates from 10% to 150% of the batched job rate (three runs per
     point), p50/p99 latency beside a plain per-call server at the same
     rates; every run checked against the batch's words;
+  * map_unmap: vkhel_vector_map + unmap latency of a 2^16-word vector,
+    pooled staging vs a staging buffer allocated per map;
   * pcie_inclusive: the headline batch with inputs and outputs in host
     memory (per-call reference API, and a pipelined batch with transfers
     on copy streams) -- reported beside `value`, never as it;
@@ -618,6 +620,38 @@ def pointwise_leg(V, ctx, ev, words=1 << 27, reps=5):
     return out
 
 
+def map_unmap_leg(V, ctx, logn=16, cycles=50):
+    """vkhel_vector_map + vkhel_vector_unmap of one 2^16-word vector
+    (src/vector.c:270-296): with the context's staging pool (default; the
+    page-locked buffer is reused) and with it off (a buffer allocated and
+    freed per cycle, as the reference stages per map).  Median microseconds
+    per cycle over `cycles` cycles after a warm-up."""
+    import ctypes
+    L = V.load()
+    n = 1 << logn
+    v = ctx.vector(n, False)
+    v.fill_uniform(1 << 60, 3)
+    want = v.read()
+    out = {"workload": f"map+unmap of 2^{logn} words", "cycles": cycles}
+    for name, pool in (("pooled_us", True), ("per_call_alloc_us", False)):
+        ctx.set_staging_pool(pool)
+        times = []
+        for k in range(cycles + 1):
+            p = ctypes.c_void_p()
+            t0 = time.perf_counter()
+            L.vkhel_vector_map(v.handle, ctypes.byref(p), 8 * n)
+            L.vkhel_vector_unmap(v.handle)
+            dt = time.perf_counter() - t0
+            if k:
+                times.append(dt)
+        out[name] = round(pct(times, 50) * 1e6, 2)
+    ctx.set_staging_pool(True)
+    assert np.array_equal(v.map_read(), want)
+    v.close()
+    out["speedup"] = round(out["per_call_alloc_us"] / out["pooled_us"], 2)
+    return out
+
+
 def pcie_inclusive(V, ctx, logn=16, primes=8, per_prime=64, chunks=8,
                    per_call_sample=32):
     """Host-resident inputs and outputs (the headline batch, n = 2^16,
@@ -962,6 +996,7 @@ def main():
         extra["pointwise"] = pointwise_leg(V, ctx, ev)
         extra["uneven_batch"], extra["serving"] = uneven_and_serving(V, ctx, ev)
         extra["pcie_inclusive"] = pcie_inclusive(V, ctx)
+        extra["map_unmap"] = map_unmap_leg(V, ctx)
 
     cpu = None if args.no_cpu else cpu_baseline(args.logn, qs, ws, V, ctx)
     tb = ctx.tile_bits(args.logn)

@@ -59,6 +59,13 @@ int vkhel_ext_vector_read(const struct vkhel_vector *, uint64_t offset,
 int vkhel_ext_vector_write(struct vkhel_vector *, uint64_t offset,
 		uint64_t count, const uint64_t *host);
 
+/* vkhel_vector_map stages through page-locked buffers from a per-context
+ * pool (reused by later maps; freed with the context).  enabled 0 frees
+ * each buffer at unmap instead (and frees the idle ones now); 1 (default)
+ * keeps them.  vkhel_ext_staging_bytes: bytes the pool holds. */
+int vkhel_ext_set_staging_pool(struct vkhel_ctx *, int enabled);
+uint64_t vkhel_ext_staging_bytes(struct vkhel_ctx *);
+
 /* ---- asynchronous transfers ---- */
 /* Page-locked host memory (null on failure); free with vkhel_ext_host_free.
  * Copies from and to it run as DMA without a staging copy and overlap

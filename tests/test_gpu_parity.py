@@ -1316,3 +1316,47 @@ def test_one_launch_two_pass_matches_split(ctx, logn, tb):
         ctx.set_tile_bits(logn, 0)
         for t in tabs:
             t.close()
+
+
+def test_map_staging_pool_reuses_buffers(ctx):
+    """vkhel_vector_map stages through the context's pool of page-locked
+    buffers: after the first cycle, map/unmap cycles allocate nothing (the
+    pool's size stays put), concurrent maps get distinct buffers, writes
+    through a mapping reach the device at unmap (src/vector.c:270-296), and
+    with the pool off idle buffers are freed."""
+    import ctypes
+    L = V.load()
+    n = 1 << 16
+    a, b = ctx.vector(n, False), ctx.vector(n, False)
+    a.fill_uniform(1 << 60, 5)
+    b.fill_uniform(1 << 60, 6)
+    xa, xb = a.read(), b.read()
+    ctx.set_staging_pool(True)
+    for _ in range(3):
+        assert np.array_equal(a.map_read(), xa)
+    held = ctx.staging_bytes
+    assert held >= 8 * n
+    for _ in range(10):
+        assert np.array_equal(a.map_read(), xa)
+        assert np.array_equal(b.map_read(), xb)
+    assert ctx.staging_bytes == held  # reused, nothing new allocated
+    # two mappings at once: distinct buffers; a write through one reaches
+    # the device at unmap
+    pa, pb = ctypes.c_void_p(), ctypes.c_void_p()
+    L.vkhel_vector_map(a.handle, ctypes.byref(pa), 8 * n)
+    L.vkhel_vector_map(b.handle, ctypes.byref(pb), 8 * n)
+    assert pa.value != pb.value
+    ha = np.ctypeslib.as_array(ctypes.cast(pa, ctypes.POINTER(ctypes.c_uint64)), shape=(n,))
+    assert np.array_equal(ha, xa)
+    ha[:] = np.arange(n, dtype=np.uint64)
+    L.vkhel_vector_unmap(a.handle)
+    L.vkhel_vector_unmap(b.handle)
+    assert np.array_equal(a.read(), np.arange(n, dtype=np.uint64))
+    assert ctx.staging_bytes >= 16 * n
+    ctx.set_staging_pool(False)
+    assert ctx.staging_bytes == 0
+    assert np.array_equal(b.map_read(), xb)
+    assert ctx.staging_bytes == 0
+    ctx.set_staging_pool(True)
+    a.close()
+    b.close()

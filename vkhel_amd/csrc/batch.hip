@@ -411,18 +411,19 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				uint64_t *tmp = sp;
 				sp += n;
 				if (polymul_fused_tile(ctx, t->c.logn)) {
-					// b's spectrum, then a's forward, the product and the
-					// inverse in one kernel (after a's forward column pass
-					// for two-pass sizes, which runs in place in res)
-					ts.fwd.push_back(make_task(*t, j.b->dev + off, tmp, false));
+					// both forwards, the product and the inverse in one
+					// kernel; two-pass sizes first run the forward column
+					// passes of a (in place in res) and b (into scratch)
 					const bool cols = t->c.logn > (uint32_t) polymul_fused_tile(ctx,
 							t->c.logn);
-					if (cols)
+					if (cols) {
 						ts.pm_col.push_back(make_task(*t, j.a->dev + off,
 								j.res->dev + off, false));
+						ts.pm_col.push_back(make_task(*t, j.b->dev + off, tmp, false));
+					}
 					NttTask pm = make_task(*t, cols ? j.res->dev + off
 							: j.a->dev + off, j.res->dev + off, true);
-					pm.in2 = tmp;
+					pm.in2 = cols ? tmp : j.b->dev + off;
 					pm.tw2 = t->fwd;
 					ts.pm.push_back(pm);
 					break;
@@ -596,12 +597,13 @@ static int launch_segments(const vkhel_ctx *ctx,
 	return VKHEL_OK;
 }
 
-// One pointwise phase: a row of workgroups per task, rows in launches of at
-// most 65535 (the grid's y limit).
+// One pointwise phase: a row of workgroups per task, enough for one
+// access per operand and thread on the longest task (capped at 1024 per
+// row), rows in launches of at most 65535 (the grid's y limit).
 static int launch_pointwise(const std::vector<PointTask> &h, const PointTask *d,
 		uint64_t maxlen, hipStream_t s) {
 	const uint64_t blocks = std::max<uint64_t>(
-			std::min<uint64_t>((maxlen / 2 + 255) / 256, 64), 1);
+			std::min<uint64_t>((maxlen / 2 + 255) / 256, 1024), 1);
 	for (size_t first = 0; first < h.size(); first += 65535) {
 		const size_t rows = std::min<size_t>(h.size() - first, 65535);
 		hipLaunchKernelGGL(k_pointwise_tasks, dim3((unsigned) blocks,
@@ -613,15 +615,16 @@ static int launch_pointwise(const std::vector<PointTask> &h, const PointTask *d,
 }
 
 // The fused polymul segments; a two-pass size's column-pass tasks form a
-// segment of the same (log2 n, class) key, with the same jobs in the same
-// order (both lists are filled in job order and sorted stably by that key).
+// segment of the same (log2 n, class) key with two tasks per product (a's
+// and b's column passes, adjacent), the same jobs in the same order (both
+// lists are filled in job order and sorted stably by that key).
 static int launch_polymul_segments(const TaskSet &ts, const DeviceTasks &dt,
 		hipStream_t s) {
 	for (const Segment &g : ts.seg_pm) {
 		const NttTask *cols = nullptr;
 		for (const Segment &c : ts.seg_pm_col) {
 			if (c.logn == g.logn && c.mc == g.mc) {
-				if (c.count != g.count)
+				if (c.count != 2 * g.count)
 					return fail(VKHEL_ERR_INVALID, "polymul plan: column segment "
 							"of %u tasks for %u products", c.count, g.count);
 				cols = dt.pm_col + c.off;

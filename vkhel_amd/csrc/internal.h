@@ -105,6 +105,17 @@ struct vkhel_ctx {
 
 	std::mutex mu;  // guards `tables` (batch and serve threads share it)
 	std::unordered_map<uint64_t, vk::DevTables *> tables;
+	// Page-locked staging buffers of vkhel_vector_map, reused: a map takes
+	// the smallest free buffer that fits (allocating only when none does),
+	// unmap returns it.  staging_pool 0: free on unmap instead.
+	struct Staging {
+		void *p;
+		size_t bytes;
+		bool busy;
+	};
+	std::mutex staging_mu;
+	std::vector<Staging> staging;
+	int staging_pool = 1;
 };
 
 struct vkhel_vector {
@@ -202,16 +213,18 @@ bool uses_handoff(const vkhel_ctx *ctx, uint32_t logn);
 // them.
 int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn);
 // Launches `ntasks` fused polynomial products on 2^tile_bits tiles:
-// pm_tasks {in, out, tw = inverse twiddles, tw2 = forward twiddles, in2 =
-// b's spectrum}.  Two-pass sizes also take col_tasks, task i the forward
-// column pass of pm task i's a (in -> out, forward twiddles), and read a
-// from out in the fused kernel; one-tile sizes take col_tasks null.
+// pm_tasks {in = a, out, tw = inverse twiddles, tw2 = forward twiddles,
+// in2 = b}.  Two-pass sizes also take col_tasks, two per product: tasks 2i
+// and 2i + 1 the forward column passes of product i's a (into its out) and
+// b (into scratch), which the pm task then names as in and in2; one-tile
+// sizes take col_tasks null and read a and b themselves.
 int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
 		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st);
-// One fused polynomial product without task arrays: the column task (two-
-// pass sizes, cols true) and the product task passed by value.
-int launch_polymul_one(const NttTask &col, const NttTask &pm, bool cols,
-		uint32_t logn, int tile_bits, int mc, hipStream_t st);
+// One fused polynomial product without task arrays: the column tasks of a
+// and b (two-pass sizes, cols true) and the product task passed by value.
+int launch_polymul_one(const NttTask &col_a, const NttTask &col_b,
+		const NttTask &pm, bool cols, uint32_t logn, int tile_bits, int mc,
+		hipStream_t st);
 // One fused transform of task k (its in2 / epilogue fields honoured), by
 // value: reads no task array.
 int launch_task_fused(const vkhel_ctx *ctx, const NttTask &k, bool inverse,

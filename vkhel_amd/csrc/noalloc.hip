@@ -69,20 +69,22 @@ static int polymul_limb(vkhel_ctx *ctx, const DevTables &t, const uint64_t *a,
 		const uint64_t *b, uint64_t *res, uint64_t *tmp, hipStream_t st) {
 	const uint32_t logn = t.c.logn;
 	const int tb = polymul_fused_tile(ctx, logn);
+	if (tb) {
+		// both forwards (after their column passes, a's in place in res and
+		// b's into tmp, for two-pass sizes), the product and the inverse in
+		// the fused kernels
+		const bool cols = logn > (uint32_t) tb;
+		const NttTask col_a = make_task(t, a, res, false);
+		const NttTask col_b = make_task(t, b, tmp, false);
+		NttTask pm = make_task(t, cols ? res : a, res, true);
+		pm.in2 = cols ? tmp : b;
+		pm.tw2 = t.fwd;
+		return launch_polymul_one(col_a, col_b, pm, cols, logn, tb,
+				modulus_class(t.c.q), st);
+	}
 	int rc = launch_ntt(ctx, t, b, tmp, 1, false, st);
 	if (rc != VKHEL_OK)
 		return rc;
-	if (tb) {
-		// a's forward (column pass in place in res for two-pass sizes), the
-		// product with b's spectrum and the inverse in the fused kernels
-		const bool cols = logn > (uint32_t) tb;
-		const NttTask col = make_task(t, a, res, false);
-		NttTask pm = make_task(t, cols ? res : a, res, true);
-		pm.in2 = tmp;
-		pm.tw2 = t.fwd;
-		return launch_polymul_one(col, pm, cols, logn, tb, modulus_class(t.c.q),
-				st);
-	}
 	rc = launch_ntt(ctx, t, a, res, 1, false, st);
 	if (rc != VKHEL_OK)
 		return rc;

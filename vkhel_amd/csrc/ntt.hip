@@ -323,11 +323,13 @@ int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn) {
 }
 
 // Fused polynomial products from device task arrays (col_tasks null for
-// one-tile sizes) or, with both arrays null, from `ucol` / `upm` for one
-// product (the kernels' uniform mode, nothing read from device memory).
-static int polymul_impl(const NttTask &ucol, const NttTask &upm,
-		const NttTask *col_tasks, const NttTask *pm_tasks, bool cols,
-		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st) {
+// one-tile sizes; otherwise two per product, a's column pass then b's) or,
+// with both arrays null, from `ucol_a`, `ucol_b` and `upm` for one product
+// (the kernels' uniform mode, nothing read from device memory).
+static int polymul_impl(const NttTask &ucol_a, const NttTask &ucol_b,
+		const NttTask &upm, const NttTask *col_tasks, const NttTask *pm_tasks,
+		bool cols, uint32_t ntasks, uint32_t logn, int tile_bits, int mc,
+		hipStream_t st) {
 	if (ntasks == 0)
 		return VKHEL_OK;
 	if (!fused_supported(logn))
@@ -347,11 +349,19 @@ static int polymul_impl(const NttTask &ucol, const NttTask &upm,
 		const uint32_t chunk = (uint32_t) std::max<uint64_t>(1,
 				std::min<uint64_t>(kChunkWords >> logn, 65535));
 		for (uint32_t first = 0; first < ntasks; first += chunk) {
-			const dim3 grid(e.tiles, std::min(chunk, ntasks - first));
-			const NttTask *ct = col_tasks ? col_tasks + first : nullptr;
+			const uint32_t count = std::min(chunk, ntasks - first);
+			const dim3 grid(e.tiles, count);
 			const NttTask *pt = pm_tasks ? pm_tasks + first : nullptr;
-			hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, ucol, ct,
-					0u);
+			// the forward column passes of a and b
+			if (col_tasks) {
+				hipLaunchKernelGGL(e.fwd_a[mc], dim3(e.tiles, 2 * count),
+						dim3(e.threads), 0, st, ucol_a, col_tasks + 2 * first, 0u);
+			} else {
+				hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, ucol_a,
+						(const NttTask *) nullptr, 0u);
+				hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, ucol_b,
+						(const NttTask *) nullptr, 0u);
+			}
 			hipLaunchKernelGGL(e.pm[mc], grid, dim3(e.threads), 0, st, upm, pt, 0u);
 			hipLaunchKernelGGL(e.inv_a[mc], grid, dim3(e.threads), 0, st, upm, pt,
 					1u);
@@ -364,14 +374,15 @@ static int polymul_impl(const NttTask &ucol, const NttTask &upm,
 int launch_polymul_tasks(const NttTask *col_tasks, const NttTask *pm_tasks,
 		uint32_t ntasks, uint32_t logn, int tile_bits, int mc, hipStream_t st) {
 	const NttTask none{};
-	return polymul_impl(none, none, col_tasks, pm_tasks, col_tasks != nullptr,
-			ntasks, logn, tile_bits, mc, st);
+	return polymul_impl(none, none, none, col_tasks, pm_tasks,
+			col_tasks != nullptr, ntasks, logn, tile_bits, mc, st);
 }
 
-int launch_polymul_one(const NttTask &col, const NttTask &pm, bool cols,
-		uint32_t logn, int tile_bits, int mc, hipStream_t st) {
-	return polymul_impl(col, pm, nullptr, nullptr, cols, 1, logn, tile_bits, mc,
-			st);
+int launch_polymul_one(const NttTask &col_a, const NttTask &col_b,
+		const NttTask &pm, bool cols, uint32_t logn, int tile_bits, int mc,
+		hipStream_t st) {
+	return polymul_impl(col_a, col_b, pm, nullptr, nullptr, cols, 1, logn,
+			tile_bits, mc, st);
 }
 
 int launch_task_fused(const vkhel_ctx *ctx, const NttTask &k, bool inverse,

@@ -395,6 +395,19 @@ __host__ __device__ __forceinline__ uint32_t tile_addr(uint32_t l,
 		jbase;
 }
 
+// The tile-local index of a round's element e is l0 | (e << F), and
+// round_base leaves bits F..F+2 of l0 clear, so the two bit fields are
+// disjoint.  Both pad() and tile_addr() act on disjoint fields additively
+// (shifts and masks with no carries between the fields), so an element's
+// LDS index and global offset are the base's plus a compile-time constant;
+// spelled out, they compile to immediate instruction offsets instead of
+// per-element address arithmetic.
+template <class G>
+__host__ __device__ constexpr uint32_t tile_off(uint32_t l) {
+	return G::CBITS >= G::KT ? l :
+		((l >> G::CBITS) << G::JSHIFT) + (l & ((1u << G::CBITS) - 1));
+}
+
 // Local index of element 0 of thread t in a round whose 3-bit field sits at
 // local bit f; element e is at l0 + (e << f).
 template <int F>
@@ -576,26 +589,29 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		if constexpr (PRE) {
 			// operands already in v
 		} else if (FIRST_ROUND && RD::DIRECT && PRODUCT && src2) {
+			const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
 			for (int e = 0; e < 8; e++) {
-				const uint32_t gi = tile_addr<G>(l0 + (e << F), jbase);
+				const uint32_t gi = g0 + tile_off<G>((uint32_t) e << F);
 				v[e] = spectra_product(gload_in<G::P>(src, gi),
 						gload_in<G::P>(src2, gi), c);
 			}
 		} else if (FIRST_ROUND && RD::DIRECT && LIFT && c.lift_add) {
+			const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				v[e] = lift_word(gload_in<G::P>(src, tile_addr<G>(l0 + (e << F),
-						jbase)), c);
-		} else {
+				v[e] = lift_word(gload_in<G::P>(src, g0 +
+						tile_off<G>((uint32_t) e << F)), c);
+		} else if (FIRST_ROUND && RD::DIRECT) {
+			const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
-			for (int e = 0; e < 8; e++) {
-				const uint32_t l = l0 + (e << F);
-				if (FIRST_ROUND && RD::DIRECT)
-					v[e] = gload_in<G::P>(src, tile_addr<G>(l, jbase));
-				else
-					v[e] = lds[pad(l)];
-			}
+			for (int e = 0; e < 8; e++)
+				v[e] = gload_in<G::P>(src, g0 + tile_off<G>((uint32_t) e << F));
+		} else {
+			const uint32_t p0 = pad(l0);
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+				v[e] = lds[p0 + pad((uint32_t) e << F)];
 		}
 		// every twiddle of the round, issued before any arithmetic (loading
 		// the next round's before the barrier instead costs registers beyond
@@ -733,19 +749,20 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 		static_assert(INV || G::P == 1, "forward final round stores directly");
 		if (active) {
 			// store addresses formed here, not held from the kernel's start
-			const uint32_t ls = round_base<F>(opaque32_here(t));
+			const uint32_t g0 = tile_addr<G>(round_base<F>(opaque32_here(t)), jbase);
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				gstore(dst, tile_addr<G>(ls + (e << F), jbase), v[e]);
+				gstore(dst, g0 + tile_off<G>((uint32_t) e << F), v[e]);
 		}
 		if constexpr (TRAIL)
 			__syncthreads();
 	} else if constexpr (LAST_ROUND) {
 		// lane-contiguous global stores staged through LDS
 		if (active) {
+			const uint32_t p0 = pad(l0);
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				lds[pad(l0 + (e << F))] = v[e];
+				lds[p0 + pad((uint32_t) e << F)] = v[e];
 		}
 		exchange_sync<!LOCAL>();
 		if (active) {
@@ -768,9 +785,10 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 			__syncthreads();
 	} else {
 		if (active) {
+			const uint32_t p0 = pad(l0);
 #pragma unroll
 			for (int e = 0; e < 8; e++)
-				lds[pad(l0 + (e << F))] = v[e];
+				lds[p0 + pad((uint32_t) e << F)] = v[e];
 		}
 		exchange_sync<!(LOCAL && round_wave_local<G, INV, R + 1>())>();
 	}
@@ -991,15 +1009,15 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, 1, TB)))) void k_ntt_handoff
 	}
 }
 
-// The middle of a polynomial product a*b fused into one kernel: a's
-// forward rounds (twiddles tw2) with canonical results kept in registers,
-// the pointwise product with b's spectrum (in2) at the same words, then the
-// inverse rounds (twiddles tw) from those registers.  It saves storing a's
-// spectrum and loading it back.  One-tile sizes run the whole product, one
-// workgroup per task (blockIdx.x); two-pass sizes run the row pass of both
-// transforms, one workgroup per (tile, task) = (blockIdx.x, blockIdx.y), on
-// input whose forward column pass is done, and the inverse column pass
-// follows.  Possible when the forward's last round and the inverse's first
+// The middle of a polynomial product a*b fused into one kernel: b's and
+// a's forward rounds (twiddles tw2; b from in2, a from in) with canonical
+// results kept in registers, their pointwise product, then the inverse
+// rounds (twiddles tw) from those registers.  Neither spectrum is stored
+// and loaded back.  One-tile sizes run the whole product, one workgroup per
+// task (blockIdx.x), reading a and b and writing the result: 24 bytes per
+// output word.  Two-pass sizes run the row pass of all three transforms,
+// one workgroup per (tile, task) = (blockIdx.x, blockIdx.y), on a and b
+// after their forward column passes, and the inverse column pass follows.  Possible when the forward's last round and the inverse's first
 // both hold their field at bit 0: thread t then holds words 8t..8t+7 in
 // both, and both are wave-local, so a wave fence orders the forward's LDS
 // reads before the inverse's first exchange writes.
@@ -1018,9 +1036,12 @@ __device__ __forceinline__ u64x2 gload2(const uint64_t *p, uint32_t i) {
 	return *(gcu128 *) ((gcchar *) p + (i << 3));
 }
 
-// one-tile sizes leave the register budget to the compiler, as k_ntt_fused
+// One-tile sizes leave the register budget to the compiler, as k_ntt_fused;
+// the row kernel holds both forward results (16 VGPRs more than a pass
+// kernel), so it runs at six waves per SIMD instead of eight (for this
+// arithmetic-bound kernel six and eight measured alike).
 constexpr int pm_waves(int logn, int tb) {
-	return logn <= tb ? 1 : pass_waves(logn, 2, tb);
+	return logn <= tb ? 1 : 6;
 }
 
 template <int LOGN, int TB, int MC>
@@ -1045,17 +1066,19 @@ __attribute__((amdgpu_waves_per_eu(pm_waves(LOGN, TB)))) void k_ntt_pm(
 		const uint32_t t = threadIdx.x;
 		const uint32_t jbase = PASS == 0 ? 0 : blockIdx.x << TB;
 		const auto seq = std::make_integer_sequence<int, G::ROUNDS>();
-		uint64_t v[8];
+		// b's forward rounds, then a's, both kept in registers in the same
+		// layout (thread t holds words 8t..8t+7 after the last round)
+		uint64_t vb[8], v[8];
+		run_rounds_v<G, false, LOGN, false, MC, kIoKeep>(seq, vb, t, k.in2,
+				nullptr, k.out, lds, jbase, jbase, k.tw2, k.c, true);
+		// (a's first round may write LDS words other waves read in b's last)
+		__syncthreads();
 		run_rounds_v<G, false, LOGN, false, MC, kIoKeep>(seq, v, t, k.in,
 				nullptr, k.out, lds, jbase, jbase, k.tw2, k.c, true);
 		if (G::THREADS >= 64 || t < (uint32_t) G::THREADS) {
-			const uint32_t l0 = tile_addr<G>(round_base<0>(t), jbase);
 #pragma unroll
-			for (int e = 0; e < 8; e += 2) {
-				const u64x2 y = gload2(k.in2, l0 + e);
-				v[e] = spectra_product(v[e], y.x, k.c);
-				v[e + 1] = spectra_product(v[e + 1], y.y, k.c);
-			}
+			for (int e = 0; e < 8; e++)
+				v[e] = spectra_product(v[e], vb[e], k.c);
 		}
 		wave_sync();
 		run_rounds_v<G, true, LOGN, false, MC, kIoPreloaded>(seq, v, t, nullptr,

@@ -170,6 +170,8 @@ void vkhel_ctx_destroy(struct vkhel_ctx *ctx) {
 	(void) hipStreamSynchronize(ctx->stream);
 	for (auto &kv : ctx->tables)
 		free_dev_tables(kv.second);
+	for (auto &b : ctx->staging)
+		(void) hipHostFree(b.p);
 	(void) hipStreamDestroy(ctx->stream);
 	(void) hipStreamDestroy(ctx->table_stream);
 	delete ctx;
@@ -315,13 +317,49 @@ struct vkhel_vector *vkhel_vector_create2(struct vkhel_ctx *ctx,
 	return vector_new(ctx, length, zero);
 }
 
+// A page-locked staging buffer of at least `bytes` from the context's pool
+// (smallest free one that fits; allocated when none does), or nullptr with
+// the error recorded.
+static void *staging_get(vkhel_ctx *ctx, size_t bytes) {
+	std::lock_guard<std::mutex> g(ctx->staging_mu);
+	vkhel_ctx::Staging *best = nullptr;
+	for (auto &b : ctx->staging)
+		if (!b.busy && b.bytes >= bytes && (!best || b.bytes < best->bytes))
+			best = &b;
+	if (best) {
+		best->busy = true;
+		return best->p;
+	}
+	void *p = nullptr;
+	if (!hip_ok(hipHostMalloc(&p, bytes, 0), "hipHostMalloc(map)"))
+		return nullptr;
+	ctx->staging.push_back(vkhel_ctx::Staging{p, bytes, true});
+	return p;
+}
+
+// Returns a staging buffer to the pool (or frees it with the pool off).
+static void staging_put(vkhel_ctx *ctx, void *p) {
+	std::lock_guard<std::mutex> g(ctx->staging_mu);
+	for (size_t i = 0; i < ctx->staging.size(); i++) {
+		if (ctx->staging[i].p != p)
+			continue;
+		if (ctx->staging_pool) {
+			ctx->staging[i].busy = false;
+		} else {
+			(void) hipHostFree(p);
+			ctx->staging.erase(ctx->staging.begin() + i);
+		}
+		return;
+	}
+}
+
 // src/vector.c:231-235
 void vkhel_vector_destroy(struct vkhel_vector *v) {
 	if (!v)
 		return;
 	(void) hipStreamSynchronize(v->ctx->stream);
 	if (v->staging)
-		(void) hipHostFree(v->staging);
+		staging_put(v->ctx, v->staging);
 	if (!v->view)
 		(void) hipFree(v->dev);
 	free(v);
@@ -356,22 +394,24 @@ void vkhel_vector_copy_from_host(struct vkhel_vector *v, const uint64_t *e) {
 // src/vector.c:270-283: a host buffer holding the device contents.  The
 // staging buffer is max(size, length*8) bytes so callers that pass a short
 // size (examples/example.c:8,32,40) still get a valid mapping; the first
-// min(size, length*8) bytes are copied from the device.
+// min(size, length*8) bytes are copied from the device.  The reference
+// allocates a staging buffer per map (src/vector.c:274-278); here it comes
+// from the context's pool of page-locked buffers, so steady-state map /
+// unmap cycles allocate nothing and copy by DMA.
 void vkhel_vector_map(struct vkhel_vector *v, void **mem, size_t size) {
 	if (!v || !mem)
 		return (void) fail(VKHEL_ERR_INVALID, "map: null argument");
 	if (v->staging)
 		return (void) fail(VKHEL_ERR_INVALID, "map: vector already mapped");
 	const size_t words = std::max<size_t>((size + 7) / 8, v->length);
-	void *p = nullptr;
-	if (!hip_ok(hipHostMalloc(&p, std::max<size_t>(words, 1) * 8, 0),
-				"hipHostMalloc(map)"))
+	void *p = staging_get(v->ctx, std::max<size_t>(words, 1) * 8);
+	if (!p)
 		return;
 	const size_t copy = std::min<size_t>(size, v->length * 8);
 	if (copy && !(hip_ok(hipMemcpyAsync(p, v->dev, copy, hipMemcpyDeviceToHost,
 						v->ctx->stream), "map copy") &&
 				hip_ok(hipStreamSynchronize(v->ctx->stream), "sync"))) {
-		(void) hipHostFree(p);
+		staging_put(v->ctx, p);
 		return;
 	}
 	v->staging = p;
@@ -386,7 +426,7 @@ void vkhel_vector_unmap(struct vkhel_vector *v) {
 	hip_ok(hipMemcpyAsync(v->dev, v->staging, v->length * 8,
 				hipMemcpyHostToDevice, v->ctx->stream), "unmap copy") &&
 		hip_ok(hipStreamSynchronize(v->ctx->stream), "sync");
-	(void) hipHostFree(v->staging);
+	staging_put(v->ctx, v->staging);
 	v->staging = nullptr;
 	v->staging_words = 0;
 }
@@ -404,6 +444,35 @@ struct vkhel_vector *vkhel_ext_vector_view(struct vkhel_vector *base,
 	v->dev = base->dev + offset;
 	v->view = true;
 	return v;
+}
+
+int vkhel_ext_set_staging_pool(struct vkhel_ctx *ctx, int enabled) {
+	if (!ctx)
+		return fail(VKHEL_ERR_INVALID, "set_staging_pool: null ctx");
+	std::lock_guard<std::mutex> g(ctx->staging_mu);
+	ctx->staging_pool = enabled ? 1 : 0;
+	if (!enabled) {
+		// free what no mapping holds
+		for (size_t i = 0; i < ctx->staging.size();) {
+			if (!ctx->staging[i].busy) {
+				(void) hipHostFree(ctx->staging[i].p);
+				ctx->staging.erase(ctx->staging.begin() + i);
+			} else {
+				i++;
+			}
+		}
+	}
+	return VKHEL_OK;
+}
+
+uint64_t vkhel_ext_staging_bytes(struct vkhel_ctx *ctx) {
+	if (!ctx)
+		return 0;
+	std::lock_guard<std::mutex> g(ctx->staging_mu);
+	uint64_t n = 0;
+	for (auto &b : ctx->staging)
+		n += b.bytes;
+	return n;
 }
 
 void *vkhel_ext_ctx_stream(struct vkhel_ctx *ctx) {

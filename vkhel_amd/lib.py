@@ -56,6 +56,8 @@ SIGNATURES = {
     "vkhel_ext_vector_view": (_p, [_p, _u64, _u64]),
     "vkhel_ext_vector_read": (_int, [_p, _u64, _u64, _pu64]),
     "vkhel_ext_vector_write": (_int, [_p, _u64, _u64, _pu64]),
+    "vkhel_ext_set_staging_pool": (_int, [_p, _int]),
+    "vkhel_ext_staging_bytes": (_u64, [_p]),
     "vkhel_ext_host_alloc": (_p, [ctypes.c_size_t]),
     "vkhel_ext_host_free": (None, [_p]),
     "vkhel_ext_vector_write_async": (_int, [_p, _u64, _u64, _pu64, _p]),

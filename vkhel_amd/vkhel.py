@@ -51,6 +51,14 @@ class Context:
     def stream(self):
         return _L().vkhel_ext_ctx_stream(self._h)
 
+    def set_staging_pool(self, enabled):
+        """vkhel_ext_set_staging_pool: reuse map staging buffers (default)."""
+        _lib.check(_L().vkhel_ext_set_staging_pool(self._h, int(bool(enabled))))
+
+    @property
+    def staging_bytes(self):
+        return int(_L().vkhel_ext_staging_bytes(self._h))
+
     def synchronize(self):
         _lib.check(_L().vkhel_ext_ctx_synchronize(self._h))
 
