@@ -126,8 +126,10 @@ enum vkhel_ntt_algo {
 int vkhel_ext_set_ntt_algo(struct vkhel_ctx *, int algo);
 
 /* ---- autotuning of the fused NTT kernels ----
- * Sizes above 2^11 run as two fused passes over tiles of 2^11, 2^12 or
- * 2^13 words (the tile size fixes the stage split and workgroup shape).
+ * Sizes above a tile run as two fused passes over tiles of 2^11, 2^12,
+ * 2^13 or 2^14 words (the tile size fixes the stage split and workgroup
+ * shape; 2^14-word tiles use 144 KiB of LDS and 1024 threads, and run
+ * n = 2^14 in one pass).
  * vkhel_ext_autotune times each compiled tile size, with both pass modes
  * (below), for every log2 n in [logn_lo, logn_hi] (3..20) on `batch`
  * synthetic polynomials (0: enough to fill the device) and keeps the
@@ -136,7 +138,7 @@ int vkhel_ext_autotune(struct vkhel_ctx *, uint32_t logn_lo, uint32_t logn_hi,
 		uint32_t batch);
 /* Tile size (log2 words) in use for log2 n, or -1 on error. */
 int vkhel_ext_tile_bits(struct vkhel_ctx *, uint32_t logn);
-/* tile_bits 11..13, or 0 for the built-in default. */
+/* tile_bits 11..14, or 0 for the built-in default. */
 int vkhel_ext_set_tile_bits(struct vkhel_ctx *, uint32_t logn, int tile_bits);
 /* Pass mode for log2 n: 0 = built-in default (split), 1 = one workgroup
  * per polynomial for both passes, 2 = one grid per pass with one workgroup

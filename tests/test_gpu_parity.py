@@ -488,7 +488,7 @@ def test_batch_direct_launch_replays(ctx):
         x.close()
 
 
-@pytest.mark.parametrize("tb", [11, 12, 13])
+@pytest.mark.parametrize("tb", [11, 12, 13, 14])
 def test_fused_polymul_tiles(ctx, tb):
     """Polymul jobs run a's forward, the product and the inverse in one
     kernel (after a's column pass for two-pass sizes): every tile size, one-
@@ -556,8 +556,8 @@ def test_batch_roundtrip_full_size(ctx):
 
 @pytest.mark.parametrize("bits", [55, 59])
 @pytest.mark.parametrize("mode", [1, 2])
-@pytest.mark.parametrize("tb", [11, 12, 13])
-@pytest.mark.parametrize("logn", [12, 13, 14, 16, 17, 18, 19, 20])
+@pytest.mark.parametrize("tb", [11, 12, 13, 14])
+@pytest.mark.parametrize("logn", [12, 13, 14, 15, 16, 17, 18, 19, 20])
 def test_tile_sizes_per_call_and_batched(ctx, logn, tb, mode, bits):
     """Every compiled tile size in both pass modes (1 = one workgroup per
     polynomial running both passes, 2 = one grid per pass, one workgroup per
@@ -1263,7 +1263,7 @@ def test_batch_with_70000_limbs(ctx):
         t.close()
 
 
-@pytest.mark.parametrize("tb", [11, 12, 13])
+@pytest.mark.parametrize("tb", [11, 12, 13, 14])
 @pytest.mark.parametrize("logn", [14, 16, 17])
 def test_one_launch_two_pass_matches_split(ctx, logn, tb):
     """Pass mode 3 (both passes in one launch, k_ntt_handoff: per-XCD ticket

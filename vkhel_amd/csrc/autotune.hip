@@ -27,7 +27,8 @@ __global__ void k_fill_residues(uint64_t *p, uint64_t n, uint64_t q,
 	}
 }
 
-static const int kTileChoices[] = {11, 12, 13};
+static const int kTileChoices[] = {11, 12, 13, 14};
+constexpr int kNumTiles = 4;
 static const int kTuneRounds = 4;
 
 }  // namespace vk
@@ -45,9 +46,9 @@ int vkhel_ext_tile_bits(struct vkhel_ctx *ctx, uint32_t logn) {
 int vkhel_ext_set_tile_bits(struct vkhel_ctx *ctx, uint32_t logn,
 		int tile_bits) {
 	if (!ctx || logn >= 64 || (tile_bits != 0 && (tile_bits < 11 ||
-					tile_bits > 13)))
+					tile_bits > 14)))
 		return fail(VKHEL_ERR_INVALID, "set_tile_bits: tile_bits must be 0 or "
-				"11..13");
+				"11..14");
 	ctx->tile_bits[logn] = tile_bits;
 	return VKHEL_OK;
 }
@@ -132,16 +133,19 @@ int vkhel_ext_autotune(struct vkhel_ctx *ctx, uint32_t logn_lo,
 		// passes (one-tile sizes have a single kernel).  Each is timed in
 		// several interleaved rounds and keeps its fastest, so a transient
 		// slowdown of the device does not decide the choice.
-		int cand_tb[6], cand_mode[6], ncand = 0;
-		for (int choice = 0; choice < 6; choice++) {
-			const int tb = kTileChoices[choice % 3];
-			const int mode = 1 + choice / 3;  // whole polynomial, then split
-			if (logn <= (uint32_t) tb && (tb != 13 || mode == 2))
-				continue;  // one-tile size: every table holds the same kernel
+		int cand_tb[2 * kNumTiles], cand_mode[2 * kNumTiles], ncand = 0;
+		for (int choice = 0; choice < 2 * kNumTiles; choice++) {
+			const int tb = kTileChoices[choice % kNumTiles];
+			const int mode = 1 + choice / kNumTiles;  // whole polynomial, then split
+			// one-tile sizes have one kernel (the same in every table that
+			// holds the size in one tile): time it once
+			const int one_tile_tb = logn <= 13 ? 13 : 14;
+			if (logn <= (uint32_t) tb && (mode == 1 || tb != one_tile_tb))
+				continue;
 			cand_tb[ncand] = tb;
 			cand_mode[ncand++] = mode;
 		}
-		float cand_ms[6];
+		float cand_ms[2 * kNumTiles];
 		for (int i = 0; i < ncand; i++)
 			cand_ms[i] = 1e30f;
 		int rc = VKHEL_OK;
@@ -218,7 +222,7 @@ int vkhel_ext_autotune_load(struct vkhel_ctx *ctx, const char *path) {
 	int tb, mode;
 	int rc = VKHEL_OK;
 	while (fscanf(f, "%u %d %d", &l, &tb, &mode) == 3) {
-		if (l >= ctx->tile_bits.size() || (tb != 0 && (tb < 11 || tb > 13)) ||
+		if (l >= ctx->tile_bits.size() || (tb != 0 && (tb < 11 || tb > 14)) ||
 				mode < 0 || mode > 3) {
 			rc = fail(VKHEL_ERR_INVALID, "autotune_load: bad entry %u %d %d", l, tb,
 					mode);

@@ -147,7 +147,8 @@ static bool fused_supported(uint32_t logn) {
 // Tile sizes (log2 words) the fused kernels are compiled for.
 static const FusedEntry &fused_entry(uint32_t logn, int tile_bits) {
 	const FusedEntry *t = tile_bits == 11 ? fused_table_kt11()
-		: tile_bits == 12 ? fused_table_kt12() : fused_table_kt13();
+		: tile_bits == 12 ? fused_table_kt12() : tile_bits == 13 ? fused_table_kt13()
+		: fused_table_kt14();
 	return t[logn - kMinFusedLogn];
 }
 
@@ -315,11 +316,19 @@ int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
 			use_split(ctx, logn), inverse, mc, st);
 }
 
+// The context's tile when the fused product kernel exists for it (not for
+// 2^14-word tiles, whose threads each run two groups of words), else the
+// largest tile where it does.
 int polymul_fused_tile(const vkhel_ctx *ctx, uint32_t logn) {
 	if (!fused_supported(logn))
 		return 0;
 	const int tb = tile_bits_for(ctx, logn);
-	return fused_entry(logn, tb).pm_ok ? tb : 0;
+	if (fused_entry(logn, tb).pm_ok)
+		return tb;
+	for (int alt = 13; alt >= 11; alt--)
+		if (fused_entry(logn, alt).pm_ok)
+			return alt;
+	return 0;
 }
 
 // Fused polynomial products from device task arrays (col_tasks null for

@@ -286,7 +286,11 @@ struct Geo {
 	// global stage bit = local bit + SHIFT
 	static constexpr int SHIFT = PASS == 1 ? K2 - CB : 0;
 	static constexpr int ROUNDS = (HI - LO + 2) / 3;
+	// threads' worth of work (8 words each), and the physical threads of
+	// a workgroup that run it (a workgroup holds at most 1024)
 	static constexpr int THREADS = 1 << (KT - 3);
+	static constexpr int PHYS = THREADS > 1024 ? 1024 : THREADS;
+	static constexpr int VT = THREADS / PHYS;
 	static constexpr int NTILES = PASS == 0 ? 1 : 1 << (LOGN - TB);
 };
 
@@ -537,9 +541,16 @@ constexpr bool round_wave_local() {
 // results in v (no stores).
 enum { kIoPreloaded = 1, kIoKeep = 2 };
 
+// Tiles of 2^14 words have 2048 threads' worth of work (8 words each) but
+// a workgroup holds at most 1024: physical thread tp then runs the round's
+// work of virtual threads tp and tp + 1024 one after the other (G::VT).
+// A physical wave always runs the same two virtual waves, each of which
+// touches only its own 512-word block in wave-local rounds, so the
+// exchange rules above hold unchanged; the synchronisations of a round
+// come after all its virtual threads.
 template <class G, bool INV, int R, int LOGN, bool TRAIL, int MC, int IO = 0>
 __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
-		uint32_t t, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
+		uint32_t tp, const uint64_t *src, const uint64_t *src2, uint64_t *dst,
 		uint64_t *lds, uint32_t jbase, uint32_t gbase, const ulonglong2 *tw,
 		const NttConsts &c, bool final) {
 	using RD = Round<G, INV, R>;
@@ -549,10 +560,13 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	constexpr bool LOCAL = round_wave_local<G, INV, R>();
 	constexpr bool PRE = FIRST_ROUND && (IO & kIoPreloaded);
 	constexpr bool KEEP = LAST_ROUND && (IO & kIoKeep);
-	const uint32_t l0 = round_base<F>(t);
-	// workgroups hold exactly G::THREADS threads, or 64 when a one-tile
-	// size needs fewer
-	const bool active = G::THREADS >= 64 || t < (uint32_t) G::THREADS;
+	constexpr uint32_t PHYS = G::PHYS;
+	constexpr int VT = G::VT;
+	static_assert(VT == 1 || !(IO & (kIoPreloaded | kIoKeep)),
+			"values kept in registers across rounds need one virtual thread");
+	// workgroups hold exactly G::PHYS threads, or 64 when a one-tile size
+	// needs fewer
+	const bool active = G::THREADS >= 64 || tp < (uint32_t) G::THREADS;
 
 	// ---- operands ----
 	// the first loads of an inverse transform may form a pointwise product
@@ -562,234 +576,250 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	if (FIRST_ROUND && !RD::DIRECT && !PRE) {
 		// lane-contiguous global loads staged through LDS
 		if (active) {
-			if (PRODUCT && src2) {
-				// four products in flight: all eight would not fit in
-				// eight waves' registers
+#pragma unroll
+			for (int i = 0; i < VT; i++) {
+				const uint32_t t = tp + i * PHYS;
+				if (PRODUCT && src2) {
+					// four products in flight: all eight would not fit in
+					// eight waves' registers
 #pragma unroll 4
-				for (int k = 0; k < 8; k++) {
-					const uint32_t gi = Staged<G>::global_at(t, k, jbase);
-					lds[Staged<G>::lds_at(t, k)] = spectra_product(
-							gload_in<G::P>(src, gi), gload_in<G::P>(src2, gi), c);
+					for (int k = 0; k < 8; k++) {
+						const uint32_t gi = Staged<G>::global_at(t, k, jbase);
+						lds[Staged<G>::lds_at(t, k)] = spectra_product(
+								gload_in<G::P>(src, gi), gload_in<G::P>(src2, gi), c);
+					}
+				} else if (LIFT && c.lift_add) {
+#pragma unroll
+					for (int k = 0; k < 8; k++)
+						lds[Staged<G>::lds_at(t, k)] = lift_word(gload_in<G::P>(src,
+								Staged<G>::global_at(t, k, jbase)), c);
+				} else {
+#pragma unroll
+					for (int k = 0; k < 8; k++)
+						lds[Staged<G>::lds_at(t, k)] = gload_in<G::P>(src,
+								Staged<G>::global_at(t, k, jbase));
 				}
-			} else if (LIFT && c.lift_add) {
-#pragma unroll
-				for (int k = 0; k < 8; k++)
-					lds[Staged<G>::lds_at(t, k)] = lift_word(gload_in<G::P>(src,
-							Staged<G>::global_at(t, k, jbase)), c);
-			} else {
-#pragma unroll
-				for (int k = 0; k < 8; k++)
-					lds[Staged<G>::lds_at(t, k)] = gload_in<G::P>(src,
-							Staged<G>::global_at(t, k, jbase));
 			}
 		}
 		exchange_sync<!LOCAL>();
 	}
-	if (active) {
-		if constexpr (PRE) {
-			// operands already in v
-		} else if (FIRST_ROUND && RD::DIRECT && PRODUCT && src2) {
-			const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
-			for (int e = 0; e < 8; e++) {
-				const uint32_t gi = g0 + tile_off<G>((uint32_t) e << F);
-				v[e] = spectra_product(gload_in<G::P>(src, gi),
-						gload_in<G::P>(src2, gi), c);
-			}
-		} else if (FIRST_ROUND && RD::DIRECT && LIFT && c.lift_add) {
-			const uint32_t g0 = tile_addr<G>(l0, jbase);
+	for (int i = 0; i < VT; i++) {
+		const uint32_t t = tp + i * PHYS;
+		const uint32_t l0 = round_base<F>(t);
+		if (active) {
+			if constexpr (PRE) {
+				// operands already in v
+			} else if (FIRST_ROUND && RD::DIRECT && PRODUCT && src2) {
+				const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
-			for (int e = 0; e < 8; e++)
-				v[e] = lift_word(gload_in<G::P>(src, g0 +
-						tile_off<G>((uint32_t) e << F)), c);
-		} else if (FIRST_ROUND && RD::DIRECT) {
-			const uint32_t g0 = tile_addr<G>(l0, jbase);
-#pragma unroll
-			for (int e = 0; e < 8; e++)
-				v[e] = gload_in<G::P>(src, g0 + tile_off<G>((uint32_t) e << F));
-		} else {
-			const uint32_t p0 = pad(l0);
-#pragma unroll
-			for (int e = 0; e < 8; e++)
-				v[e] = lds[p0 + pad((uint32_t) e << F)];
-		}
-		// every twiddle of the round, issued before any arithmetic (loading
-		// the next round's before the barrier instead costs registers beyond
-		// eight waves per SIMD and measured no faster)
-		load_round_tw<G, INV, R, LOGN>(w, t, gbase, tw);
-	}
-
-	// ---- butterflies ----
-	if (active) {
-		const LazyK lk = lazy_consts(c);
-		// inverse with deferred sum reductions (q < kNoReduceQ): stages
-		// present in this round, and the bound its inputs enter with (below
-		// q at the transform's first round, below 8q after any other)
-		constexpr bool NR = MC == kClassSmall;
-		constexpr bool MID = MC == kClassMid;
-		constexpr bool LAZY_INV = INV && NR;
-		constexpr int ACTIVE = (F >= RD::BLO && F < RD::BHI ? 1 : 0) |
-			(F + 1 >= RD::BLO && F + 1 < RD::BHI ? 2 : 0) |
-			(F + 2 >= RD::BLO && F + 2 < RD::BHI ? 4 : 0);
-		constexpr int BIN = FIRST_ROUND && G::P != 1 ? 1 : 8;
-		constexpr int TOP = LOGN - 1 - G::SHIFT;  // local bit of the last stage
-		constexpr bool HAS_LAST = INV && TOP >= F && TOP <= F + 2 &&
-			TOP >= RD::BLO && TOP < RD::BHI;
-#pragma unroll
-		for (int k = 0; k < 3; k++) {
-			const int d = INV ? k : 2 - k;
-			const int lb = F + d;
-			if (lb < RD::BLO || lb >= RD::BHI)
-				continue;
-			const int b = lb + G::SHIFT;
-			if (!INV && b == LOGN - 1) {
-				// Inputs below q (the usual case) take the lazy butterfly;
-				// a wave holding any larger word follows the reference's
-				// exact reduction.
-				bool canon = true;
+				for (int e = 0; e < 8; e++) {
+					const uint32_t gi = g0 + tile_off<G>((uint32_t) e << F);
+					v[e] = spectra_product(gload_in<G::P>(src, gi),
+							gload_in<G::P>(src2, gi), c);
+				}
+			} else if (FIRST_ROUND && RD::DIRECT && LIFT && c.lift_add) {
+				const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
 				for (int e = 0; e < 8; e++)
-					canon &= v[e] < c.q;
-				if (__all(canon)) {
+					v[e] = lift_word(gload_in<G::P>(src, g0 +
+							tile_off<G>((uint32_t) e << F)), c);
+			} else if (FIRST_ROUND && RD::DIRECT) {
+				const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
-					for (int e = 0; e < 4; e++)
-						bf_fwd_first_canon<MC>(v[e], v[e + 4], w[0], lk);
+				for (int e = 0; e < 8; e++)
+					v[e] = gload_in<G::P>(src, g0 + tile_off<G>((uint32_t) e << F));
+			} else {
+				const uint32_t p0 = pad(l0);
+#pragma unroll
+				for (int e = 0; e < 8; e++)
+					v[e] = lds[p0 + pad((uint32_t) e << F)];
+			}
+			// every twiddle of the round, issued before any arithmetic
+			// (loading the next round's before the barrier instead costs
+			// registers beyond eight waves per SIMD and measured no faster)
+			load_round_tw<G, INV, R, LOGN>(w, t, gbase, tw);
+		}
+
+		// ---- butterflies ----
+		if (active) {
+			const LazyK lk = lazy_consts(c);
+			// inverse with deferred sum reductions (q < kNoReduceQ): stages
+			// present in this round, and the bound its inputs enter with
+			// (below q at the transform's first round, below 8q after any
+			// other)
+			constexpr bool NR = MC == kClassSmall;
+			constexpr bool MID = MC == kClassMid;
+			constexpr bool LAZY_INV = INV && NR;
+			constexpr int ACTIVE = (F >= RD::BLO && F < RD::BHI ? 1 : 0) |
+				(F + 1 >= RD::BLO && F + 1 < RD::BHI ? 2 : 0) |
+				(F + 2 >= RD::BLO && F + 2 < RD::BHI ? 4 : 0);
+			constexpr int BIN = FIRST_ROUND && G::P != 1 ? 1 : 8;
+			constexpr int TOP = LOGN - 1 - G::SHIFT;  // local bit of the last stage
+			constexpr bool HAS_LAST = INV && TOP >= F && TOP <= F + 2 &&
+				TOP >= RD::BLO && TOP < RD::BHI;
+#pragma unroll
+			for (int k = 0; k < 3; k++) {
+				const int d = INV ? k : 2 - k;
+				const int lb = F + d;
+				if (lb < RD::BLO || lb >= RD::BHI)
+					continue;
+				const int b = lb + G::SHIFT;
+				if (!INV && b == LOGN - 1) {
+					// Inputs below q (the usual case) take the lazy butterfly;
+					// a wave holding any larger word follows the reference's
+					// exact reduction.
+					bool canon = true;
+#pragma unroll
+					for (int e = 0; e < 8; e++)
+						canon &= v[e] < c.q;
+					if (__all(canon)) {
+#pragma unroll
+						for (int e = 0; e < 4; e++)
+							bf_fwd_first_canon<MC>(v[e], v[e + 4], w[0], lk);
+					} else {
+#pragma unroll
+						for (int e = 0; e < 4; e++)
+							bf_fwd_first(v[e], v[e + 4], w[0], c);
+					}
+				} else if (INV && b == LOGN - 1) {
+#pragma unroll
+					for (int e = 0; e < 4; e++) {
+						if constexpr (LAZY_INV)
+							bf_inv_last_nr(v[e], v[e + 4], c.q << log2_exact(
+									inv_bound(BIN, ACTIVE, d - 1, e + 4)), c, lk);
+						else if constexpr (MID)
+							bf_inv_last_nr(v[e], v[e + 4], lk.four_q, c, lk);
+						else
+							bf_inv_last(v[e], v[e + 4], c, lk);
+					}
 				} else {
 #pragma unroll
-					for (int e = 0; e < 4; e++)
-						bf_fwd_first(v[e], v[e + 4], w[0], c);
-				}
-			} else if (INV && b == LOGN - 1) {
+					for (int gi = 0; gi < (4 >> d); gi++) {
 #pragma unroll
-				for (int e = 0; e < 4; e++) {
-					if constexpr (LAZY_INV)
-						bf_inv_last_nr(v[e], v[e + 4], c.q << log2_exact(
-								inv_bound(BIN, ACTIVE, d - 1, e + 4)), c, lk);
-					else if constexpr (MID)
-						bf_inv_last_nr(v[e], v[e + 4], lk.four_q, c, lk);
-					else
-						bf_inv_last(v[e], v[e + 4], c, lk);
-				}
-			} else {
-#pragma unroll
-				for (int gi = 0; gi < (4 >> d); gi++) {
-#pragma unroll
-					for (int li = 0; li < (1 << d); li++) {
-						const int e = (gi << (d + 1)) | li;
-						if (LAZY_INV)
-							bf_inv_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
-									c.q << log2_exact(inv_bound(BIN, ACTIVE, d - 1,
-											e | (1 << d))), lk);
-						else if (INV && b == 0)
-							bf_inv_first<MID>(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
-									lk);
-						else if (INV && MID)
-							bf_inv_mid(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
-						else if (INV)
-							bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
-						else if (NR)
-							bf_fwd_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
-						else if (MID)
-							bf_fwd_mid(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
-						else
-							bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+						for (int li = 0; li < (1 << d); li++) {
+							const int e = (gi << (d + 1)) | li;
+							if (LAZY_INV)
+								bf_inv_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
+										c.q << log2_exact(inv_bound(BIN, ACTIVE, d - 1,
+												e | (1 << d))), lk);
+							else if (INV && b == 0)
+								bf_inv_first<MID>(v[e], v[e | (1 << d)], w[tw_slot(d, gi)],
+										lk);
+							else if (INV && MID)
+								bf_inv_mid(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+							else if (INV)
+								bf_inv(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+							else if (NR)
+								bf_fwd_nr(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+							else if (MID)
+								bf_fwd_mid(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+							else
+								bf_fwd(v[e], v[e | (1 << d)], w[tw_slot(d, gi)], lk);
+						}
 					}
 				}
 			}
-		}
-		if constexpr (LAZY_INV && !HAS_LAST) {
-			// back below 8q for the next round (whose registers each mix
-			// values of every register here): halve each bound by
-			// conditional subtractions of (bound / 2) q, ..., 8q.  Within a
-			// round the bounds then reach 64q (sums) and the products'
-			// inputs stay below 64q < 2^63.
-			const LazyInvK ik = lazy_inv_consts(c);
-#pragma unroll
-			for (int e = 0; e < 8; e++) {
-#pragma unroll
-				for (int h = 5; h >= 3; h--) {
-					if ((1 << h) < inv_bound(BIN, ACTIVE, 2, e))
-						v[e] = csub_n(v[e], c.q << h, ik.n[h]);
-				}
-			}
-		}
-		if constexpr (HAS_LAST) {
-			// the transform's results, canonical: the optional epilogue
-			if (c.post_p) {
-#pragma unroll
-				for (int e = 0; e < 8; e++)
-					v[e] = post_center(v[e], c);
-			}
-		}
-		if (!INV && LAST_ROUND && final) {
-			if (NR || MID) {
-				// from below 81q (small class) or 8q (mid): a quotient
-				// estimate off by at most one (small_reduce_consts), then one
-				// subtraction
+			if constexpr (LAZY_INV && !HAS_LAST) {
+				// back below 8q for the next round (whose registers each mix
+				// values of every register here): halve each bound by
+				// conditional subtractions of (bound / 2) q, ..., 8q.  Within
+				// a round the bounds then reach 64q (sums) and the products'
+				// inputs stay below 64q < 2^63.
+				const LazyInvK ik = lazy_inv_consts(c);
 #pragma unroll
 				for (int e = 0; e < 8; e++) {
-					const uint32_t qe = __umulhi((uint32_t) (v[e] >> c.red_shift),
-							c.red_m);
-					v[e] = csub_n(v[e] + (uint64_t) qe * lk.nq, c.q, lk.nq);
+#pragma unroll
+					for (int h = 5; h >= 3; h--) {
+						if ((1 << h) < inv_bound(BIN, ACTIVE, 2, e))
+							v[e] = csub_n(v[e], c.q << h, ik.n[h]);
+					}
 				}
-			} else {
+			}
+			if constexpr (HAS_LAST) {
+				// the transform's results, canonical: the optional epilogue
+				if (c.post_p) {
+#pragma unroll
+					for (int e = 0; e < 8; e++)
+						v[e] = post_center(v[e], c);
+				}
+			}
+			if (!INV && LAST_ROUND && final) {
+				if (NR || MID) {
+					// from below 81q (small class) or 8q (mid): a quotient
+					// estimate off by at most one (small_reduce_consts), then
+					// one subtraction
+#pragma unroll
+					for (int e = 0; e < 8; e++) {
+						const uint32_t qe = __umulhi((uint32_t) (v[e] >> c.red_shift),
+								c.red_m);
+						v[e] = csub_n(v[e] + (uint64_t) qe * lk.nq, c.q, lk.nq);
+					}
+				} else {
+#pragma unroll
+					for (int e = 0; e < 8; e++)
+						v[e] = csub_n(csub_n(v[e], lk.two_q, lk.n_two_q), c.q, lk.nq);
+				}
+			}
+		}
+
+		// ---- results of this virtual thread ----
+		if constexpr (KEEP) {
+			// results stay in v
+		} else if constexpr (LAST_ROUND && RD::DIRECT) {
+			// (a forward transform's final round holds its field at bit 0
+			// and always stores through LDS, where the epilogue is applied)
+			static_assert(INV || G::P == 1, "forward final round stores directly");
+			if (active) {
+				// store addresses formed here, not held from the kernel's start
+				const uint32_t g0 = tile_addr<G>(round_base<F>(opaque32_here(t)),
+						jbase);
 #pragma unroll
 				for (int e = 0; e < 8; e++)
-					v[e] = csub_n(csub_n(v[e], lk.two_q, lk.n_two_q), c.q, lk.nq);
+					gstore(dst, g0 + tile_off<G>((uint32_t) e << F), v[e]);
+			}
+		} else {
+			// to LDS: the next round's exchange, or the staged store below
+			if (active) {
+				const uint32_t p0 = pad(l0);
+#pragma unroll
+				for (int e = 0; e < 8; e++)
+					lds[p0 + pad((uint32_t) e << F)] = v[e];
 			}
 		}
 	}
 
-	// ---- results ----
+	// ---- results: synchronisation and staged stores ----
 	if constexpr (KEEP) {
 		// results stay in v
 	} else if constexpr (LAST_ROUND && RD::DIRECT) {
-		// (a forward transform's final round holds its field at bit 0 and
-		// always stores through LDS, where the epilogue is applied)
-		static_assert(INV || G::P == 1, "forward final round stores directly");
-		if (active) {
-			// store addresses formed here, not held from the kernel's start
-			const uint32_t g0 = tile_addr<G>(round_base<F>(opaque32_here(t)), jbase);
-#pragma unroll
-			for (int e = 0; e < 8; e++)
-				gstore(dst, g0 + tile_off<G>((uint32_t) e << F), v[e]);
-		}
 		if constexpr (TRAIL)
 			__syncthreads();
 	} else if constexpr (LAST_ROUND) {
 		// lane-contiguous global stores staged through LDS
-		if (active) {
-			const uint32_t p0 = pad(l0);
-#pragma unroll
-			for (int e = 0; e < 8; e++)
-				lds[p0 + pad((uint32_t) e << F)] = v[e];
-		}
 		exchange_sync<!LOCAL>();
 		if (active) {
-			const uint32_t ts = opaque32_here(t);
-			if (!INV && final && c.scale_w) {
 #pragma unroll
-				for (int k = 0; k < 8; k++) {
-					const uint32_t gi = Staged<G>::global_at(ts, k, jbase);
-					gstore(dst, gi, scale_word(gload(src2, gi),
-							lds[Staged<G>::lds_at(ts, k)], c));
+			for (int i = 0; i < VT; i++) {
+				const uint32_t ts = opaque32_here(tp + i * PHYS);
+				if (!INV && final && c.scale_w) {
+#pragma unroll
+					for (int k = 0; k < 8; k++) {
+						const uint32_t gi = Staged<G>::global_at(ts, k, jbase);
+						gstore(dst, gi, scale_word(gload(src2, gi),
+								lds[Staged<G>::lds_at(ts, k)], c));
+					}
+				} else {
+#pragma unroll
+					for (int k = 0; k < 8; k++)
+						gstore(dst, Staged<G>::global_at(ts, k, jbase),
+								lds[Staged<G>::lds_at(ts, k)]);
 				}
-			} else {
-#pragma unroll
-				for (int k = 0; k < 8; k++)
-					gstore(dst, Staged<G>::global_at(ts, k, jbase),
-							lds[Staged<G>::lds_at(ts, k)]);
 			}
 		}
 		if constexpr (TRAIL)
 			__syncthreads();
 	} else {
-		if (active) {
-			const uint32_t p0 = pad(l0);
-#pragma unroll
-			for (int e = 0; e < 8; e++)
-				lds[p0 + pad((uint32_t) e << F)] = v[e];
-		}
 		exchange_sync<!(LOCAL && round_wave_local<G, INV, R + 1>())>();
 	}
 }
@@ -834,7 +864,8 @@ __device__ __forceinline__ void run_pass(const uint64_t *src,
 template <int LOGN, int TB>
 struct FusedShape {
 	static constexpr int KT = LOGN <= TB ? LOGN : TB;
-	static constexpr int THREADS = (1 << (KT - 3)) < 64 ? 64 : 1 << (KT - 3);
+	static constexpr int THREADS = (1 << (KT - 3)) < 64 ? 64
+		: (1 << (KT - 3)) > 1024 ? 1024 : 1 << (KT - 3);
 	static constexpr int LDS_WORDS = (1 << KT) + (1 << (KT - 3));
 };
 
@@ -875,8 +906,10 @@ __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
 // Waves per SIMD the pass kernels are compiled for: eight (at most 64
 // VGPRs), which every shape fits without spilling except pass 1 of 2^20 on
 // 2^13-word tiles (ten stages over eight columns), which needs about 96.
+// (2^14-word tiles take 144 KiB of LDS, one workgroup of 16 waves per CU:
+// four waves per SIMD, so registers are not the limit there.)
 constexpr int pass_waves(int logn, int pass, int tb) {
-	return logn == 20 && pass == 1 && tb == 13 ? 5 : 8;
+	return tb == 14 ? 4 : logn == 20 && pass == 1 && tb == 13 ? 5 : 8;
 }
 
 template <int LOGN, int PASS, bool INV, int TB, int MC>
@@ -1027,8 +1060,8 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, 1, TB)))) void k_ntt_handoff
 template <int LOGN, int TB>
 constexpr bool pm_fusable() {
 	using G = Geo<LOGN, LOGN <= TB ? 0 : 2, TB>;
-	return (LOGN <= TB || G::K2 == 9) && Round<G, false, G::ROUNDS - 1>::F == 0 &&
-		Round<G, true, 0>::F == 0;
+	return G::VT == 1 && (LOGN <= TB || G::K2 == 9) &&
+		Round<G, false, G::ROUNDS - 1>::F == 0 && Round<G, true, 0>::F == 0;
 }
 
 // Words i, i + 1 of p (i even).
@@ -1128,5 +1161,6 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 const FusedEntry *fused_table_kt11();
 const FusedEntry *fused_table_kt12();
 const FusedEntry *fused_table_kt13();
+const FusedEntry *fused_table_kt14();
 
 }  // namespace vk
