@@ -142,7 +142,8 @@ int vkhel_ext_tile_bits(struct vkhel_ctx *, uint32_t logn);
 int vkhel_ext_set_tile_bits(struct vkhel_ctx *, uint32_t logn, int tile_bits);
 /* Pass mode for log2 n: 0 = built-in default (split), 1 = one workgroup
  * per polynomial for both passes, 2 = one grid per pass with one workgroup
- * per tile (split).  The autotuner sets it too. */
+ * per tile (split), 3 = both passes in one launch (see below).  The
+ * autotuner sets it too. */
 int vkhel_ext_pass_mode(struct vkhel_ctx *, uint32_t logn);
 int vkhel_ext_set_pass_mode(struct vkhel_ctx *, uint32_t logn, int mode);
 /* Pass mode 3 runs both passes of a two-pass size in one launch (batches
@@ -151,7 +152,20 @@ int vkhel_ext_set_pass_mode(struct vkhel_ctx *, uint32_t logn, int mode);
  * ahead of its second, and hands the polynomial from one pass to the other
  * through its own L2.  0 (default): derived from the occupancy. */
 int vkhel_ext_set_handoff_ahead(struct vkhel_ctx *, int polys);
-/* Tuned choices as text lines "log2n tile_bits pass_mode". */
+/* Split passes (mode 2) can also be shaped per size: the stage split (0 =
+ * the default: pass 1 takes the top log2 n - 9 stages where rows of 512
+ * words fit, else floor(log2 n / 2); 1 = one column stage more, or one
+ * fewer where that does not fit; sizes with no alternative ignore it) and
+ * the row pass's own tile (0 = the tile_bits tile; ignored where that tile
+ * would split the stages differently).  vkhel_ext_autotune tunes tile and
+ * pass mode, then the split and the row-pass tile, timing each candidate
+ * through the batch path. */
+int vkhel_ext_split(struct vkhel_ctx *, uint32_t logn);
+int vkhel_ext_set_split(struct vkhel_ctx *, uint32_t logn, int split);
+int vkhel_ext_tile_bits2(struct vkhel_ctx *, uint32_t logn);
+int vkhel_ext_set_tile_bits2(struct vkhel_ctx *, uint32_t logn, int tile_bits);
+/* Tuned choices as text lines "log2n tile_bits pass_mode split tile_bits2"
+ * (lines of the first three fields load too). */
 int vkhel_ext_autotune_save(struct vkhel_ctx *, const char *path);
 int vkhel_ext_autotune_load(struct vkhel_ctx *, const char *path);
 

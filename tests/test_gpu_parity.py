@@ -603,21 +603,80 @@ def test_tile_sizes_per_call_and_batched(ctx, logn, tb, mode, bits):
         t.close()
 
 
+def _tuning(ctx, l):
+    return (ctx.tile_bits(l), ctx.pass_mode(l), ctx.split(l), ctx.tile_bits2(l))
+
+
+def _untune(ctx, l):
+    ctx.set_tile_bits(l, 0)
+    ctx.set_pass_mode(l, 0)
+    ctx.set_split(l, 0)
+    ctx.set_tile_bits2(l, 0)
+
+
 def test_autotune_choices_and_cache(ctx, tmp_path):
-    ctx.autotune(12, 14, 64)
-    chosen = {l: (ctx.tile_bits(l), ctx.pass_mode(l)) for l in (12, 13, 14)}
-    assert all(tb in (11, 12, 13) and m in (1, 2)
-               for tb, m in chosen.values()), chosen
+    ctx.autotune(12, 16, 64)
+    chosen = {l: _tuning(ctx, l) for l in (12, 13, 14, 15, 16)}
+    assert all(tb in (11, 12, 13, 14) and m in (1, 2, 3) and sp in (0, 1) and
+               tb2 in (0, 11, 12, 13, 14) for tb, m, sp, tb2 in chosen.values()), chosen
     path = tmp_path / "tune.txt"
     ctx.autotune_save(path)
     for l in chosen:
-        ctx.set_tile_bits(l, 0)
-        ctx.set_pass_mode(l, 0)
+        _untune(ctx, l)
     ctx.autotune_load(path)
-    assert {l: (ctx.tile_bits(l), ctx.pass_mode(l)) for l in chosen} == chosen
+    assert {l: _tuning(ctx, l) for l in chosen} == chosen
+    # files of three fields per line (tile, pass mode) still load
+    old = tmp_path / "old.txt"
+    old.write_text("16 12 2\n14 11 1\n")
+    ctx.autotune_load(old)
+    assert _tuning(ctx, 16) == (12, 2, 0, 0) and _tuning(ctx, 14) == (11, 1, 0, 0)
     for l in chosen:
-        ctx.set_tile_bits(l, 0)
-        ctx.set_pass_mode(l, 0)
+        _untune(ctx, l)
+
+
+@pytest.mark.parametrize("logn,tb,sp,tb2", [
+    (14, 12, 1, 0), (14, 11, 0, 13), (14, 13, 1, 12), (15, 12, 1, 14),
+    (16, 12, 1, 0), (16, 13, 0, 11), (16, 11, 0, 14), (16, 12, 1, 13),
+    (17, 12, 1, 0), (17, 13, 1, 14), (17, 12, 0, 13), (19, 14, 1, 0),
+    (19, 13, 0, 12)])
+def test_split_and_row_pass_tile_choices(ctx, logn, tb, sp, tb2):
+    """The alternative stage split and a row-pass tile of its own give the
+    same words as the oracle, single calls and batches, both directions."""
+    n = 1 << logn
+    q = _prime(logn, 55)
+    w = O.root_2n(q, n)
+    t = V.NttTables(n, q, w)
+    ctx.set_tile_bits(logn, tb)
+    ctx.set_pass_mode(logn, 2)
+    ctx.set_split(logn, sp)
+    ctx.set_tile_bits2(logn, tb2)
+    try:
+        rng = np.random.default_rng(logn * 1000 + tb * 10 + sp + tb2)
+        npoly = 24 if logn <= 17 else 3
+        xs = [rng.integers(0, q, n, dtype=np.uint64) for _ in range(npoly)]
+        vs = [vec(ctx, z) for z in xs]
+        fb, ib = V.Batch(ctx), V.Batch(ctx)
+        for v in vs:
+            fb.add(V.JOB_NTT_FWD, v, None, v, [t])
+            ib.add(V.JOB_NTT_INV, v, None, v, [t])
+        fb.run()
+        for i in (0, npoly - 1):
+            assert np.array_equal(vs[i].read(), O.ntt_forward(xs[i], q, w)), i
+        ib.run()
+        for z, v in zip(xs, vs):
+            assert np.array_equal(v.read(), z)
+        a = vec(ctx, xs[0])
+        a.forward_transform(a, t)
+        assert np.array_equal(a.read(), O.ntt_forward(xs[0], q, w))
+        a.inverse_transform(a, t)
+        assert np.array_equal(a.read(), xs[0])
+        for v in vs + [a]:
+            v.close()
+        fb.close()
+        ib.close()
+    finally:
+        _untune(ctx, logn)
+        t.close()
 
 
 # ---------------- asynchronous transfers ----------------

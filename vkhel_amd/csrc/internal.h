@@ -95,6 +95,11 @@ struct vkhel_ctx {
 	int ntt_algo = VKHEL_NTT_AUTO;
 	// Tuned tile size (log2 words) per log2 n; 0 = default.
 	std::vector<int> tile_bits = std::vector<int>(64, 0);
+	// Tuned tile of a two-pass size's row pass (0: tile_bits), and stage
+	// split (0: the default split, 1: the alternative, ntt_fused.h
+	// alt_k1), per log2 n.
+	std::vector<int> tile_bits2 = std::vector<int>(64, 0);
+	std::vector<int> split = std::vector<int>(64, 0);
 	// Tuned pass mode per log2 n: 0 = by batch size, 1 = whole polynomial
 	// per workgroup, 2 = one grid per pass (one workgroup per tile), 3 = both
 	// passes in one persistent launch (batches only).
@@ -208,6 +213,14 @@ uint64_t handoff_state_bytes(uint32_t ntasks);
 // True when transforms of log2 n run both passes in one launch on ctx
 // (pass mode 3 at a two-pass size).
 bool uses_handoff(const vkhel_ctx *ctx, uint32_t logn);
+// After a one-launch transform of npoly polynomials of log2 n on ctx (its
+// tile) has completed: its state shows every ticket done and no wait given
+// up.  Reads the state back synchronously.
+bool handoff_state_ok(const vkhel_ctx *ctx, const uint32_t *state,
+		uint32_t npoly, uint32_t logn);
+// Stages of pass 1 for log2 n on 2^tb tiles with split choice s (0 for
+// one-tile sizes).
+int fused_split_k1(uint32_t logn, int tb, int s);
 // Fused polynomial products (k_ntt_pm): the tile size (log2 words) they
 // run on for log2 n on this context, 0 when that size and tile do not allow
 // them.

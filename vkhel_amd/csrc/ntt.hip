@@ -24,6 +24,7 @@
 #include <mutex>
 #include <unordered_map>
 #include <utility>
+#include <vector>
 
 #include "internal.h"
 #include "modarith.h"
@@ -196,15 +197,38 @@ int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn) {
 // of 65535.
 constexpr uint64_t kChunkWords = 1ull << 24;
 
+// The kernels of a two-pass size on a context: pass 1 (columns) from the
+// tile_bits table, pass 2 (rows) from the tile_bits2 one, both with the
+// context's stage split.  Both passes must split the stages alike; where
+// the second tile's table would not, pass 2 takes the first tile.
+struct PassPlan {
+	const FusedEntry *e1, *e2;
+	int split;
+};
+
+static PassPlan pass_plan(const vkhel_ctx *ctx, uint32_t logn, int tile_bits) {
+	const int s = logn < ctx->split.size() ? ctx->split[logn] : 0;
+	const FusedEntry &e1 = fused_entry(logn, tile_bits);
+	const int tb2 = logn < ctx->tile_bits2.size() ? ctx->tile_bits2[logn] : 0;
+	const FusedEntry *e2 = tb2 ? &fused_entry(logn, tb2) : &e1;
+	if (e2->tiles <= 1 || e2->k1[s] != e1.k1[s])
+		e2 = &e1;
+	return PassPlan{&e1, e2, s};
+}
+
 // Launches `ntasks` transforms of one size and direction: either through
 // the device task array `dev_tasks` or, when it is null, `uniform` and the
 // ntasks - 1 polynomials contiguous after it.  Split mode runs the two
-// passes as separate grids of one workgroup per tile.
-static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
-		uint32_t ntasks, uint32_t logn, int tile_bits, bool split, bool inverse,
-		int mc, hipStream_t st) {
+// passes as separate grids of one workgroup per tile (pass_plan's
+// kernels).
+static int launch_fused_impl(const vkhel_ctx *ctx, const NttTask &uniform,
+		const NttTask *dev_tasks, uint32_t ntasks, uint32_t logn, int tile_bits,
+		bool split, bool inverse, int mc, hipStream_t st) {
 	const FusedEntry &e = fused_entry(logn, tile_bits);
 	if (split && e.tiles > 1) {
+		const PassPlan pp = pass_plan(ctx, logn, tile_bits);
+		const FusedEntry &e1 = *pp.e1, &e2 = *pp.e2;
+		const int s = pp.split;
 		const uint32_t chunk = (uint32_t) std::max<uint64_t>(1,
 				std::min<uint64_t>(kChunkWords >> logn, 65535));
 		for (uint32_t first = 0; first < ntasks; first += chunk) {
@@ -220,11 +244,18 @@ static int launch_fused_impl(const NttTask &uniform, const NttTask *dev_tasks,
 				if (u.in2)
 					u.in2 += off;
 			}
-			const dim3 grid(e.tiles, count);
-			hipLaunchKernelGGL(inverse ? e.inv_b[mc] : e.fwd_a[mc], grid,
-					dim3(e.threads), 0, st, u, d, 0u);
-			hipLaunchKernelGGL(inverse ? e.inv_a[mc] : e.fwd_b[mc], grid,
-					dim3(e.threads), 0, st, u, d, 1u);
+			// forward: columns, then rows; inverse: rows, then columns
+			if (!inverse) {
+				hipLaunchKernelGGL(e1.fwd_a[s][mc], dim3(e1.tiles, count),
+						dim3(e1.threads), 0, st, u, d, 0u);
+				hipLaunchKernelGGL(e2.fwd_b[s][mc], dim3(e2.tiles, count),
+						dim3(e2.threads), 0, st, u, d, 1u);
+			} else {
+				hipLaunchKernelGGL(e2.inv_b[s][mc], dim3(e2.tiles, count),
+						dim3(e2.threads), 0, st, u, d, 0u);
+				hipLaunchKernelGGL(e1.inv_a[s][mc], dim3(e1.tiles, count),
+						dim3(e1.threads), 0, st, u, d, 1u);
+			}
 		}
 	} else {
 		hipLaunchKernelGGL(inverse ? e.inv[mc] : e.fwd[mc],
@@ -263,9 +294,34 @@ uint64_t handoff_state_bytes(uint32_t ntasks) {
 	return 4ull * handoff_state_words(ntasks);
 }
 
+// (the one-launch kernels exist for the default split and one tile for
+// both passes)
+int fused_split_k1(uint32_t logn, int tb, int s) {
+	if (!fused_supported(logn) || tb < 11 || tb > 14 || s < 0 || s > 1)
+		return 0;
+	return fused_entry(logn, tb).k1[s];
+}
+
+bool handoff_state_ok(const vkhel_ctx *ctx, const uint32_t *state,
+		uint32_t npoly, uint32_t logn) {
+	std::vector<uint32_t> h(kHandoffCounts);
+	if (hipMemcpy(h.data(), state, 4 * h.size(), hipMemcpyDeviceToHost) !=
+			hipSuccess)
+		return false;
+	const uint32_t tiles = handoff_tiles(logn, tile_bits_for(ctx, logn));
+	if (h[handoff_err_word()] != 0)
+		return false;
+	for (uint32_t x = 0; x < kXcds; x++)
+		if (h[handoff_done_word(x)] != handoff_queue_tickets(npoly, x, tiles))
+			return false;
+	return true;
+}
+
 bool uses_handoff(const vkhel_ctx *ctx, uint32_t logn) {
 	return fused_supported(logn) && pass_mode_of(ctx, logn) == 3 &&
-		fused_entry(logn, tile_bits_for(ctx, logn)).tiles > 1;
+		fused_entry(logn, tile_bits_for(ctx, logn)).tiles > 1 &&
+		ctx->split[logn] == 0 && (ctx->tile_bits2[logn] == 0 ||
+			ctx->tile_bits2[logn] == tile_bits_for(ctx, logn));
 }
 
 // Both passes of `ntasks` two-pass transforms in one persistent launch
@@ -312,7 +368,7 @@ int launch_fused_tasks(const vkhel_ctx *ctx, const NttTask *dev_tasks,
 		return launch_handoff(ctx, e, dev_tasks, ntasks, inverse, mc,
 				handoff_state, st);
 	const NttTask none{};
-	return launch_fused_impl(none, dev_tasks, ntasks, logn, tb,
+	return launch_fused_impl(ctx, none, dev_tasks, ntasks, logn, tb,
 			use_split(ctx, logn), inverse, mc, st);
 }
 
@@ -363,16 +419,16 @@ static int polymul_impl(const NttTask &ucol_a, const NttTask &ucol_b,
 			const NttTask *pt = pm_tasks ? pm_tasks + first : nullptr;
 			// the forward column passes of a and b
 			if (col_tasks) {
-				hipLaunchKernelGGL(e.fwd_a[mc], dim3(e.tiles, 2 * count),
+				hipLaunchKernelGGL(e.fwd_a[0][mc], dim3(e.tiles, 2 * count),
 						dim3(e.threads), 0, st, ucol_a, col_tasks + 2 * first, 0u);
 			} else {
-				hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, ucol_a,
+				hipLaunchKernelGGL(e.fwd_a[0][mc], grid, dim3(e.threads), 0, st, ucol_a,
 						(const NttTask *) nullptr, 0u);
-				hipLaunchKernelGGL(e.fwd_a[mc], grid, dim3(e.threads), 0, st, ucol_b,
+				hipLaunchKernelGGL(e.fwd_a[0][mc], grid, dim3(e.threads), 0, st, ucol_b,
 						(const NttTask *) nullptr, 0u);
 			}
 			hipLaunchKernelGGL(e.pm[mc], grid, dim3(e.threads), 0, st, upm, pt, 0u);
-			hipLaunchKernelGGL(e.inv_a[mc], grid, dim3(e.threads), 0, st, upm, pt,
+			hipLaunchKernelGGL(e.inv_a[0][mc], grid, dim3(e.threads), 0, st, upm, pt,
 					1u);
 		}
 	}
@@ -399,7 +455,7 @@ int launch_task_fused(const vkhel_ctx *ctx, const NttTask &k, bool inverse,
 	const uint32_t logn = k.c.logn;
 	if (!fused_supported(logn))
 		return fail(VKHEL_ERR_INVALID, "fused NTT: unsupported log2 n = %u", logn);
-	return launch_fused_impl(k, nullptr, 1, logn, tile_bits_for(ctx, logn),
+	return launch_fused_impl(ctx, k, nullptr, 1, logn, tile_bits_for(ctx, logn),
 			use_split(ctx, logn), inverse, modulus_class(k.c.q), st);
 }
 
@@ -410,7 +466,8 @@ int launch_ntt(vkhel_ctx *ctx, const DevTables &t, const uint64_t *in,
 	const uint32_t logn = t.c.logn;
 	if (ctx->ntt_algo == VKHEL_NTT_RADIX2 || !fused_supported(logn))
 		return launch_radix2(t, in, out, batch, inverse, s);
-	return launch_fused_impl(make_task(t, in, out, inverse), nullptr, batch, logn,
+	return launch_fused_impl(ctx, make_task(t, in, out, inverse), nullptr, batch,
+			logn,
 			tile_bits_for(ctx, logn), use_split(ctx, logn), inverse,
 			modulus_class(t.c.q), s);
 }

@@ -27,7 +27,9 @@ typedef void (*HandoffKernel)(NttTask, const NttTask *, uint32_t, uint32_t *,
 // two-pass sizes), and launch shape.
 struct FusedEntry {
 	FusedKernel fwd[3], inv[3];
-	PassKernel fwd_a[3], fwd_b[3], inv_b[3], inv_a[3];
+	// [split][class]: split 0 takes the top ntt_k1 stages in pass 1, split 1
+	// the top alt_k1 (the same kernels as split 0 where there is none)
+	PassKernel fwd_a[2][3], fwd_b[2][3], inv_b[2][3], inv_a[2][3];
 	// the middle of a polynomial product (k_ntt_pm; pm_ok: this size and
 	// tile allow it)
 	PassKernel pm[3];
@@ -36,6 +38,7 @@ struct FusedEntry {
 	HandoffKernel fwd_h[3], inv_h[3];
 	int threads;
 	int tiles;  // tiles per pass (1: one-tile size, passes unused)
+	int k1[2];  // stages of pass 1 per split choice (0 for one-tile sizes)
 };
 
 constexpr int kMaxFusedLogn = 20;
@@ -271,10 +274,30 @@ constexpr int ntt_k1(int logn, int tile_bits) {
 			? logn - 9 : logn / 2;
 }
 
-template <int LOGN, int PASS, int TB>
+// A stage split is usable on tiles of 2^tb words when pass 1's tiles keep
+// column runs of at least 16 words and a pass-2 row fits in one tile.
+constexpr bool split_ok(int logn, int tb, int k1) {
+	return k1 >= 1 && tb - k1 >= 4 && logn - k1 <= tb;
+}
+
+// The alternative split the tuner may pick (the `split` knob, 1): one
+// column stage more than ntt_k1, else one fewer; -1 where neither fits.
+constexpr int alt_k1(int logn, int tb) {
+	return logn <= tb ? -1
+		: split_ok(logn, tb, ntt_k1(logn, tb) + 1) ? ntt_k1(logn, tb) + 1
+		: split_ok(logn, tb, ntt_k1(logn, tb) - 1) ? ntt_k1(logn, tb) - 1 : -1;
+}
+
+// Stage split of split choice s (0: ntt_k1, 1: alt_k1, or ntt_k1 where no
+// alternative exists).
+constexpr int split_k1(int logn, int tb, int s) {
+	return s == 1 && alt_k1(logn, tb) > 0 ? alt_k1(logn, tb) : ntt_k1(logn, tb);
+}
+
+template <int LOGN, int PASS, int TB, int SPLIT = 0>
 struct Geo {
 	static constexpr int P = PASS;
-	static constexpr int K1 = ntt_k1(LOGN, TB);
+	static constexpr int K1 = split_k1(LOGN, TB, SPLIT);
 	static constexpr int K2 = LOGN - K1;
 	static constexpr int KT = PASS == 0 ? LOGN : TB;
 	static constexpr int CB = TB - K1;  // pass 1 columns per tile
@@ -912,7 +935,7 @@ constexpr int pass_waves(int logn, int pass, int tb) {
 	return tb == 14 ? 4 : logn == 20 && pass == 1 && tb == 13 ? 5 : 8;
 }
 
-template <int LOGN, int PASS, bool INV, int TB, int MC>
+template <int LOGN, int PASS, bool INV, int TB, int MC, int SPLIT = 0>
 __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
 __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass(
 		NttTask uniform, const NttTask *tasks, uint32_t second) {
@@ -929,7 +952,7 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 			if (k.in2)
 				k.in2 += off;
 		}
-		using G = Geo<LOGN, PASS, TB>;
+		using G = Geo<LOGN, PASS, TB, SPLIT>;
 		const uint32_t i = blockIdx.x;
 		const uint32_t jbase = PASS == 1 ? i << G::CB : i << TB;
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
@@ -1119,6 +1142,14 @@ __attribute__((amdgpu_waves_per_eu(pm_waves(LOGN, TB)))) void k_ntt_pm(
 	}
 }
 
+// The pass kernels of one (size, pass, direction, tile, split) for the
+// three modulus classes; the alternative split is instantiated only where
+// it exists (ALT_SPLIT: 1 there, else 0, naming the default kernels).
+#define PASS_ROW(LG, PS, INV, TB, SP) \
+	{k_ntt_pass<LG, PS, INV, TB, 0, SP>, k_ntt_pass<LG, PS, INV, TB, 1, SP>, \
+		k_ntt_pass<LG, PS, INV, TB, 2, SP>}
+#define ALT_SPLIT(LG, TB) (alt_k1(LG, TB) > 0 ? 1 : 0)
+
 // Kernel table for tile size TB, entry i serving log2 n = i + 3.
 template <int TB, int... L>
 constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
@@ -1130,18 +1161,14 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 		{k_ntt_fused<L + kMinFusedLogn, true, TB, 0>,
 			k_ntt_fused<L + kMinFusedLogn, true, TB, 1>,
 			k_ntt_fused<L + kMinFusedLogn, true, TB, 2>},
-		{k_ntt_pass<L + kMinFusedLogn, 1, false, TB, 0>,
-			k_ntt_pass<L + kMinFusedLogn, 1, false, TB, 1>,
-			k_ntt_pass<L + kMinFusedLogn, 1, false, TB, 2>},
-		{k_ntt_pass<L + kMinFusedLogn, 2, false, TB, 0>,
-			k_ntt_pass<L + kMinFusedLogn, 2, false, TB, 1>,
-			k_ntt_pass<L + kMinFusedLogn, 2, false, TB, 2>},
-		{k_ntt_pass<L + kMinFusedLogn, 2, true, TB, 0>,
-			k_ntt_pass<L + kMinFusedLogn, 2, true, TB, 1>,
-			k_ntt_pass<L + kMinFusedLogn, 2, true, TB, 2>},
-		{k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 0>,
-			k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 1>,
-			k_ntt_pass<L + kMinFusedLogn, 1, true, TB, 2>},
+		{PASS_ROW(L + kMinFusedLogn, 1, false, TB, 0),
+			PASS_ROW(L + kMinFusedLogn, 1, false, TB, ALT_SPLIT(L + kMinFusedLogn, TB))},
+		{PASS_ROW(L + kMinFusedLogn, 2, false, TB, 0),
+			PASS_ROW(L + kMinFusedLogn, 2, false, TB, ALT_SPLIT(L + kMinFusedLogn, TB))},
+		{PASS_ROW(L + kMinFusedLogn, 2, true, TB, 0),
+			PASS_ROW(L + kMinFusedLogn, 2, true, TB, ALT_SPLIT(L + kMinFusedLogn, TB))},
+		{PASS_ROW(L + kMinFusedLogn, 1, true, TB, 0),
+			PASS_ROW(L + kMinFusedLogn, 1, true, TB, ALT_SPLIT(L + kMinFusedLogn, TB))},
 		{k_ntt_pm<L + kMinFusedLogn, TB, 0>,
 			k_ntt_pm<L + kMinFusedLogn, TB, 1>,
 			k_ntt_pm<L + kMinFusedLogn, TB, 2>},
@@ -1153,7 +1180,9 @@ constexpr std::array<FusedEntry, sizeof...(L)> make_fused_table(
 			k_ntt_handoff<L + kMinFusedLogn, true, TB, 1>,
 			k_ntt_handoff<L + kMinFusedLogn, true, TB, 2>},
 		FusedShape<L + kMinFusedLogn, TB>::THREADS,
-		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0)}...};
+		1 << (L + kMinFusedLogn > TB ? L + kMinFusedLogn - TB : 0),
+		{ntt_k1(L + kMinFusedLogn, TB),
+			split_k1(L + kMinFusedLogn, TB, 1)}}...};
 }
 
 // Per-tile-size tables (ntt_fused_kt*.hip).  Sizes n <= 2^TB use the

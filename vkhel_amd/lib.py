@@ -75,6 +75,10 @@ SIGNATURES = {
     "vkhel_ext_pass_mode": (_int, [_p, _u32]),
     "vkhel_ext_set_pass_mode": (_int, [_p, _u32, _int]),
     "vkhel_ext_set_handoff_ahead": (_int, [_p, _int]),
+    "vkhel_ext_split": (_int, [_p, _u32]),
+    "vkhel_ext_set_split": (_int, [_p, _u32, _int]),
+    "vkhel_ext_tile_bits2": (_int, [_p, _u32]),
+    "vkhel_ext_set_tile_bits2": (_int, [_p, _u32, _int]),
     "vkhel_ext_autotune_save": (_int, [_p, ctypes.c_char_p]),
     "vkhel_ext_autotune_load": (_int, [_p, ctypes.c_char_p]),
     "vkhel_server_create": (_p, [_p, _p]),

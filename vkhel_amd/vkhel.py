@@ -85,6 +85,24 @@ class Context:
     def set_pass_mode(self, logn, mode):
         _lib.check(_L().vkhel_ext_set_pass_mode(self._h, logn, mode))
 
+    def split(self, logn):
+        """vkhel_ext_split: stage split choice of log2 n (0 default, 1 alt)."""
+        s = _L().vkhel_ext_split(self._h, logn)
+        _lib.check()
+        return s
+
+    def set_split(self, logn, split):
+        _lib.check(_L().vkhel_ext_set_split(self._h, logn, split))
+
+    def tile_bits2(self, logn):
+        """vkhel_ext_tile_bits2: the row pass's tile of log2 n (0: same)."""
+        tb = _L().vkhel_ext_tile_bits2(self._h, logn)
+        _lib.check()
+        return tb
+
+    def set_tile_bits2(self, logn, tile_bits):
+        _lib.check(_L().vkhel_ext_set_tile_bits2(self._h, logn, tile_bits))
+
     def set_handoff_ahead(self, polys):
         """vkhel_ext_set_handoff_ahead: look-ahead of pass mode 3's schedule
         (0: from the occupancy)."""
