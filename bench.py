@@ -877,8 +877,8 @@ def traffic_from_profile(logn, npoly, tile_bits):
 
 
 def modulus_class(q):
-    """The kernels' modulus class: 2 below 2^56, 1 below 2^60, else 0."""
-    return 2 if q < (1 << 56) else 1 if q < (1 << 60) else 0
+    """The kernels' modulus class: 2 below 2^55, 1 below 2^60, else 0."""
+    return 2 if q < (1 << 55) else 1 if q < (1 << 60) else 0
 
 
 def fwd_kernel_names(logn, tile_bits, pass_mode, mc):

@@ -135,6 +135,54 @@ def test_final_reduction_quotient_is_exact_or_one_less(q):
         assert v // q - 1 <= qe <= v // q, (v, qe)
 
 
+def _mad_u64_u32(a, b, c):
+    # v_mad_u64_u32: a, b 32-bit, c 64-bit, result mod 2^64
+    assert a < 1 << 32 and b < 1 << 32 and c < 1 << 64
+    return (a * b + c) % (1 << 64)
+
+
+def _mad_i32_i24(a, b, c):
+    # v_mad_i32_i24: the low 24 bits of a and b as signed numbers, their
+    # product plus c, mod 2^32
+    def s24(x):
+        x &= (1 << 24) - 1
+        return x - (1 << 24) if x >> 23 else x
+    return (s24(a) * s24(b) + c) % (1 << 32)
+
+
+def _sub_small_multiple(x, k, q):
+    # the two device instructions of modarith.h sub_small_multiple
+    nq = (1 << 64) - q
+    lo = _mad_u64_u32(k, nq & 0xFFFFFFFF, x)
+    hi = _mad_i32_i24(k, nq >> 32, lo >> 32)
+    return hi << 32 | (lo & 0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("q", [5, 97, 65537, (1 << 50) - 27, 1125899906826241,
+                               (1 << 55) - 55, (1 << 55) - 1,
+                               (1 << 54) + 1, (1 << 32) + 15])
+def test_sub_small_multiple_is_exact(q):
+    """The forward's final reduction takes k*q off v (k <= 80, v < 81q) with
+    one 64-bit and one 24-bit multiply-add, which is exact for odd q below
+    2^55 (the small class): checked against x - k*q in exact integers,
+    including q whose high word is 2^23 - 1, the 24-bit limit."""
+    assert q % 2 == 1 and q < 1 << 55
+    rng = np.random.default_rng(q % 997)
+    cases = [(int(x), int(k)) for x, k in zip(
+        rng.integers(0, 81 * q, 4000, dtype=np.uint64) if 81 * q < 1 << 64
+        else [], rng.integers(0, 81, 4000))]
+    cases += [(k * q + d, k) for k in range(81) for d in (0, 1, q - 1)]
+    cases += [(81 * q - 1, 80), (0, 0), (q, 1)]
+    for x, k in cases:
+        if k * q > x:
+            continue  # the estimate never exceeds floor(v / q)
+        assert _sub_small_multiple(x, k, q) == x - k * q, (x, k)
+    # one step past the limit: a high word of 2^23 no longer fits 24 bits
+    q = (1 << 55) + 1
+    x, k = 80 * q + 5, 80
+    assert _sub_small_multiple(x, k, q) != x - k * q
+
+
 def test_tables_root_check():
     n = 1 << 8
     q = O.primes_ntt(1 << 40, 2 * n, 1)[0]

@@ -171,7 +171,10 @@ __host__ __device__ __forceinline__ uint32_t tw_pos(uint32_t m, uint32_t i) {
 // so values entering stage s >= 1 stay below (4s + 1) q, below 81q < 2^63
 // after the last stage of n = 2^20; the inverse defers its sum reductions
 // (bounds below 64q).  The 63-bit Shoup product accepts all of these.
-constexpr uint64_t kNoReduceQ = 1ull << 56;
+// (2^55 rather than 2^56: the high word of such a q is below 2^23, which
+// the final reduction's 24-bit multiply needs, modarith.h
+// sub_small_multiple.)
+constexpr uint64_t kNoReduceQ = 1ull << 55;
 // Below this bound (and from kNoReduceQ up) the mid class: forward values
 // kept below 8q with one reduction per butterfly, inverse below 4q, both
 // with products below 4q (8q < 2^63).

@@ -124,6 +124,20 @@ __device__ __forceinline__ uint64_t elemfma_word(uint64_t x, uint64_t y,
 	return s >= q ? q - s : s;
 }
 
+// x - k*q modulo 2^64 for k < 2^23 and q < 2^55, given nq = 2^64 - q.
+// With q = q1*2^32 + q0 (q odd, so q0 != 0), nq's low word is 2^32 - q0
+// and its high word 2^32 - 1 - q1, which as a signed 24-bit number is
+// -(q1 + 1) (q1 < 2^23): one 64-bit multiply-add forms x + k*(2^32 - q0),
+// one 24-bit multiply-add the high word's - k*(q1 + 1).  (The general
+// product k*nq takes two 32x32 multiplies and register moves.)
+__device__ __forceinline__ uint64_t sub_small_multiple(uint64_t x, uint32_t k,
+		uint64_t nq) {
+	const uint64_t lo = (uint64_t) k * (uint32_t) nq + x;
+	const uint32_t hi = (uint32_t) (lo >> 32) +
+		(uint32_t) __mul24((int) k, (int) (uint32_t) (nq >> 32));
+	return ((uint64_t) hi << 32) | (uint32_t) lo;
+}
+
 // Reference 64-bit Barrett (elemmul.comp:36-47, elemgtsub.comp:36-47):
 // mu = floor(2^(bits+62) / q), shift = bits - 2, one subtraction.  Exact
 // (canonical) for every 64-bit x once bits >= 3.

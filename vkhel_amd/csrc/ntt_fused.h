@@ -42,6 +42,7 @@ struct FusedEntry {
 };
 
 constexpr int kMaxFusedLogn = 20;
+
 constexpr int kMinFusedLogn = 3;
 constexpr int kFusedEntries = kMaxFusedLogn - kMinFusedLogn + 1;
 
@@ -768,10 +769,19 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 				}
 			}
 			if (!INV && LAST_ROUND && final) {
-				if (NR || MID) {
-					// from below 81q (small class) or 8q (mid): a quotient
-					// estimate off by at most one (small_reduce_consts), then
+				if (NR) {
+					// from below 81q: a quotient estimate qe <= 80, off by at
+					// most one (small_reduce_consts), its multiple of q taken
+					// off with a 24-bit high word (q < kNoReduceQ = 2^55), then
 					// one subtraction
+#pragma unroll
+					for (int e = 0; e < 8; e++) {
+						const uint32_t qe = __umulhi((uint32_t) (v[e] >> c.red_shift),
+								c.red_m);
+						v[e] = csub_n(sub_small_multiple(v[e], qe, lk.nq), c.q, lk.nq);
+					}
+				} else if (MID) {
+					// from below 8q: the same estimate, a full product by q
 #pragma unroll
 					for (int e = 0; e < 8; e++) {
 						const uint32_t qe = __umulhi((uint32_t) (v[e] >> c.red_shift),
