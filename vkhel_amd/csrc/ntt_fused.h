@@ -902,10 +902,22 @@ struct FusedShape {
 	static constexpr int LDS_WORDS = (1 << KT) + (1 << (KT - 3));
 };
 
+// Waves per SIMD a whole-transform kernel is compiled for.  One-tile sizes
+// up to 2^13 words fit eight waves per SIMD in LDS (2^13: two 1024-thread
+// workgroups of 72 KiB per CU; 2^12: four of 36 KiB), which holds when the
+// kernel keeps to 64 VGPRs; left to itself the compiler gives the inverse
+// 72, and one workgroup fewer fits per CU.  A 2^14-word tile (144 KiB)
+// holds one workgroup per CU, four waves per SIMD, and two-pass sizes (the
+// whole polynomial per workgroup, pass mode 1) are left to the compiler.
+constexpr int fused_waves(int logn, int tb) {
+	return logn <= 13 && logn <= tb ? 8 : 1;
+}
+
 // One workgroup per task.  With `tasks` null, workgroup i transforms the
 // i-th polynomial after `uniform` (same tables, contiguous polynomials).
 template <int LOGN, bool INV, int TB, int MC>
-__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS)) void k_ntt_fused(
+__global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
+__attribute__((amdgpu_waves_per_eu(fused_waves(LOGN, TB)))) void k_ntt_fused(
 		NttTask uniform, const NttTask *tasks) {
 	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
 	NttTask k;
