@@ -1114,12 +1114,14 @@ __device__ __forceinline__ u64x2 gload2(const uint64_t *p, uint32_t i) {
 	return *(gcu128 *) ((gcchar *) p + (i << 3));
 }
 
-// One-tile sizes leave the register budget to the compiler, as k_ntt_fused;
-// the row kernel holds both forward results (16 VGPRs more than a pass
-// kernel), so it runs at six waves per SIMD instead of eight (for this
-// arithmetic-bound kernel six and eight measured alike).
+// One-tile sizes leave the register budget to the compiler (it takes 123-128
+// VGPRs; any lower cap spills); the row kernel holds both forward results
+// (16 VGPRs more than a pass kernel) and runs at five waves per SIMD, where
+// it needs no scratch (87 VGPRs).  At six it spilled 52 bytes per thread
+// and measured 5-6% slower on 2^27-word batches of products at n = 2^15 to
+// 2^17 on MI355X.
 constexpr int pm_waves(int logn, int tb) {
-	return logn <= tb ? 1 : 6;
+	return logn <= tb ? 1 : 5;
 }
 
 template <int LOGN, int TB, int MC>
