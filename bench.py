@@ -881,13 +881,44 @@ def modulus_class(q):
     return 2 if q < (1 << 55) else 1 if q < (1 << 60) else 0
 
 
-def fwd_kernel_names(logn, tile_bits, pass_mode, mc):
+def _ntt_k1(logn, tb):
+    # the library's default stage split (ntt_fused.h ntt_k1, split_ok, alt_k1)
+    if logn <= tb:
+        return 0
+    k = logn - 9
+    return k if k >= 1 and tb - k >= 4 and k < logn // 2 else logn // 2
+
+
+def _split_ok(logn, tb, k1):
+    return k1 >= 1 and tb - k1 >= 4 and logn - k1 <= tb
+
+
+def _alt_k1(logn, tb):
+    if logn <= tb:
+        return -1
+    k = _ntt_k1(logn, tb)
+    return k + 1 if _split_ok(logn, tb, k + 1) else k - 1 if _split_ok(logn, tb, k - 1) else -1
+
+
+def fwd_kernel_names(logn, tile_bits, pass_mode, mc, split=0, tile_bits2=0):
     """The dispatches of one forward transform at this size and tuning for
-    moduli of class mc."""
+    moduli of class mc (the last template argument of a pass kernel is 1 for
+    the alternative stage split where the size has one)."""
     if logn <= tile_bits or pass_mode == 1:
         return [f"k_ntt_fused<{logn}, false, {tile_bits}, {mc}>"]
-    return [f"k_ntt_pass<{logn}, 1, false, {tile_bits}, {mc}>",
-            f"k_ntt_pass<{logn}, 2, false, {tile_bits}, {mc}>"]
+    if pass_mode == 3:
+        return [f"k_ntt_handoff<{logn}, false, {tile_bits}, {mc}>"]
+    sp = 1 if split == 1 and _alt_k1(logn, tile_bits) > 0 else 0
+    k1 = _alt_k1(logn, tile_bits) if sp else _ntt_k1(logn, tile_bits)
+    tb2 = tile_bits2 or tile_bits
+    if tb2 != tile_bits:
+        k1b = (_alt_k1(logn, tb2) if split == 1 and _alt_k1(logn, tb2) > 0
+               else _ntt_k1(logn, tb2))
+        if logn <= tb2 or k1b != k1:
+            tb2 = tile_bits
+    sp2 = 1 if split == 1 and _alt_k1(logn, tb2) > 0 else 0
+    return [f"k_ntt_pass<{logn}, 1, false, {tile_bits}, {mc}, {sp}>",
+            f"k_ntt_pass<{logn}, 2, false, {tb2}, {mc}, {sp2}>"]
 
 
 def main():
@@ -1024,7 +1055,9 @@ def main():
                      "traffic": traffic,
                      "kernel": "forward NTT of the batch: " + " + ".join(
                          fwd_kernel_names(args.logn, tb, ctx.pass_mode(args.logn),
-                                          modulus_class(max(qs)))),
+                                          modulus_class(max(qs)),
+                                          ctx.split(args.logn),
+                                          ctx.tile_bits2(args.logn))),
                      "kernel_ms": round(r["fwd_ms"], 4),
                      "inverse_kernel_ms": round(r["inv_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes,

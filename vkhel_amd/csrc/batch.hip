@@ -302,6 +302,19 @@ static PointTask center_task(const uint64_t *a, uint64_t *out, uint64_t n,
 	return p;
 }
 
+// Workgroups of 256 threads that give a task of `len` words one two-word
+// access per thread (at most 1024: longer tasks loop).
+static uint64_t pointwise_blocks(uint64_t len) {
+	return std::max<uint64_t>(std::min<uint64_t>((len / 2 + 255) / 256, 1024), 1);
+}
+
+// Pointwise tasks in the order their launches take them: longest first
+// (they are independent, so any order gives the same words).
+static void order_pointwise(std::vector<PointTask> &h) {
+	std::stable_sort(h.begin(), h.end(), [](const PointTask &x,
+			const PointTask &y) { return x.len > y.len; });
+}
+
 int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 		uint64_t *scratch, TaskSet &ts) {
 	ts = TaskSet();
@@ -345,7 +358,6 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 					q, barrett_mu(q), ql / 2, ql % q, (uint32_t) (bitlen(q) - 2),
 					kPtRescale, w, shoup63_factor(w, q)};
 				ts.post.push_back(p);
-				ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
 				break;
 			}
 			case VKHEL_JOB_RESCALE_NTT: {
@@ -376,14 +388,12 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				PointTask lift{dropped, nullptr, j.res->dev + off, n, q, barrett_mu(q),
 					ql / 2, ql, (uint32_t) (bitlen(q) - 2), kPtGtSub};
 				ts.post.push_back(lift);
-				ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
 				ts.fwd2.push_back(make_task(*t, j.res->dev + off, j.res->dev + off,
 						false));
 				PointTask p{j.a->dev + off, j.res->dev + off, j.res->dev + off, n, q,
 					barrett_mu(q), ~0ull, 0, (uint32_t) (bitlen(q) - 2), kPtRescale, w,
 					shoup63_factor(w, q)};
 				ts.post2.push_back(p);
-				ts.post2_maxlen = std::max<uint64_t>(ts.post2_maxlen, n);
 				break;
 			}
 			case VKHEL_JOB_INTT_CENTER: {
@@ -396,7 +406,6 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 					ts.inv.push_back(inv);
 					ts.post.push_back(center_task(j.res->dev + off, j.res->dev + off, n,
 							q, pm));
-					ts.post_maxlen = std::max<uint64_t>(ts.post_maxlen, n);
 					break;
 				}
 				inv.c.post_p = pm;
@@ -440,7 +449,6 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 					p.b = tmp;
 					p.out = j.res->dev + off;
 					ts.pt.push_back(p);
-					ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
 				}
 				ts.inv.push_back(inv);
 				break;
@@ -489,11 +497,11 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 				break;
 			}
 			}
-			if (j.op >= VKHEL_JOB_MUL && j.op != VKHEL_JOB_INTT_CENTER &&
-					j.op != VKHEL_JOB_RESCALE && j.op != VKHEL_JOB_RESCALE_NTT)
-				ts.pt_maxlen = std::max<uint64_t>(ts.pt_maxlen, n);
 		}
 	}
+	order_pointwise(ts.pt);
+	order_pointwise(ts.post);
+	order_pointwise(ts.post2);
 	ts.seg_fwd = segment(ts.fwd);
 	ts.seg_inv = segment(ts.inv);
 	ts.seg_fwd2 = segment(ts.fwd2);
@@ -597,19 +605,23 @@ static int launch_segments(const vkhel_ctx *ctx,
 	return VKHEL_OK;
 }
 
-// One pointwise phase: a row of workgroups per task, enough for one
-// access per operand and thread on the longest task (capped at 1024 per
-// row), rows in launches of at most 65535 (the grid's y limit).
+// One pointwise phase over tasks ordered by order_pointwise: a row of
+// workgroups per task, in launches of tasks whose lengths need within 4x
+// the same number of workgroups (each launch sized for its longest task),
+// so short tasks do not carry the long ones' empty workgroups; rows in
+// launches of at most 65535 (the grid's y limit).
 static int launch_pointwise(const std::vector<PointTask> &h, const PointTask *d,
-		uint64_t maxlen, hipStream_t s) {
-	const uint64_t blocks = std::max<uint64_t>(
-			std::min<uint64_t>((maxlen / 2 + 255) / 256, 1024), 1);
-	for (size_t first = 0; first < h.size(); first += 65535) {
-		const size_t rows = std::min<size_t>(h.size() - first, 65535);
+		hipStream_t s) {
+	for (size_t i = 0; i < h.size();) {
+		const uint64_t blocks = pointwise_blocks(h[i].len);
+		size_t j = i + 1;
+		while (j < h.size() && j - i < 65535 && pointwise_blocks(h[j].len) * 4 >= blocks)
+			j++;
 		hipLaunchKernelGGL(k_pointwise_tasks, dim3((unsigned) blocks,
-				(unsigned) rows), dim3(256), 0, s, d + first);
+				(unsigned) (j - i)), dim3(256), 0, s, d + i);
 		if (!hip_ok(hipGetLastError(), "pointwise launch"))
 			return VKHEL_ERR_DEVICE;
+		i = j;
 	}
 	return VKHEL_OK;
 }
@@ -644,15 +656,15 @@ int enqueue_tasks(const vkhel_ctx *ctx, const TaskSet &ts,
 	if (rc == VKHEL_OK)
 		rc = launch_polymul_segments(ts, dt, s);
 	if (rc == VKHEL_OK)
-		rc = launch_pointwise(ts.pt, dt.pt, ts.pt_maxlen, s);
+		rc = launch_pointwise(ts.pt, dt.pt, s);
 	if (rc == VKHEL_OK)
 		rc = launch_segments(ctx, ts.seg_inv, ts.inv, dt.inv, true, dt, s);
 	if (rc == VKHEL_OK)
-		rc = launch_pointwise(ts.post, dt.post, ts.post_maxlen, s);
+		rc = launch_pointwise(ts.post, dt.post, s);
 	if (rc == VKHEL_OK)
 		rc = launch_segments(ctx, ts.seg_fwd2, ts.fwd2, dt.fwd2, false, dt, s);
 	if (rc == VKHEL_OK)
-		rc = launch_pointwise(ts.post2, dt.post2, ts.post2_maxlen, s);
+		rc = launch_pointwise(ts.post2, dt.post2, s);
 	return rc;
 }
 

@@ -63,12 +63,10 @@ struct TaskSet {
 	std::vector<NttTask> pm_col, pm;
 	std::vector<Segment> seg_pm_col, seg_pm;
 	std::vector<PointTask> pt, post;  // before / after the inverse phase
-	uint64_t pt_maxlen = 0, post_maxlen = 0;
 	// after `post`: forward transforms, then pointwise tasks
 	std::vector<NttTask> fwd2;
 	std::vector<Segment> seg_fwd2;
 	std::vector<PointTask> post2;
-	uint64_t post2_maxlen = 0;
 	uint64_t handoff_words = 0;  // state words of all one-launch segments
 };
 
