@@ -563,7 +563,9 @@ constexpr bool round_wave_local() {
 // IO flags of a sequence of rounds: the first round's operands are
 // already in v in its own layout (no loads), and the last round leaves its
 // results in v (no stores).
-enum { kIoPreloaded = 1, kIoKeep = 2 };
+// kIoStreamIn: the first round's global loads carry the nontemporal hint
+// whatever the pass.
+enum { kIoPreloaded = 1, kIoKeep = 2, kIoStreamIn = 4 };
 
 // Tiles of 2^14 words have 2048 threads' worth of work (8 words each) but
 // a workgroup holds at most 1024: physical thread tp then runs the round's
@@ -584,6 +586,8 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 	constexpr bool LOCAL = round_wave_local<G, INV, R>();
 	constexpr bool PRE = FIRST_ROUND && (IO & kIoPreloaded);
 	constexpr bool KEEP = LAST_ROUND && (IO & kIoKeep);
+	// pass whose load flavour the first round's global loads take
+	constexpr int LP = (IO & kIoStreamIn) ? 0 : G::P;
 	constexpr uint32_t PHYS = G::PHYS;
 	constexpr int VT = G::VT;
 	static_assert(VT == 1 || !(IO & (kIoPreloaded | kIoKeep)),
@@ -610,17 +614,17 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 					for (int k = 0; k < 8; k++) {
 						const uint32_t gi = Staged<G>::global_at(t, k, jbase);
 						lds[Staged<G>::lds_at(t, k)] = spectra_product(
-								gload_in<G::P>(src, gi), gload_in<G::P>(src2, gi), c);
+								gload_in<LP>(src, gi), gload_in<LP>(src2, gi), c);
 					}
 				} else if (LIFT && c.lift_add) {
 #pragma unroll
 					for (int k = 0; k < 8; k++)
-						lds[Staged<G>::lds_at(t, k)] = lift_word(gload_in<G::P>(src,
+						lds[Staged<G>::lds_at(t, k)] = lift_word(gload_in<LP>(src,
 								Staged<G>::global_at(t, k, jbase)), c);
 				} else {
 #pragma unroll
 					for (int k = 0; k < 8; k++)
-						lds[Staged<G>::lds_at(t, k)] = gload_in<G::P>(src,
+						lds[Staged<G>::lds_at(t, k)] = gload_in<LP>(src,
 								Staged<G>::global_at(t, k, jbase));
 				}
 			}
@@ -639,20 +643,20 @@ __device__ __forceinline__ void run_round(uint64_t v[8], ulonglong2 w[7],
 #pragma unroll
 				for (int e = 0; e < 8; e++) {
 					const uint32_t gi = g0 + tile_off<G>((uint32_t) e << F);
-					v[e] = spectra_product(gload_in<G::P>(src, gi),
-							gload_in<G::P>(src2, gi), c);
+					v[e] = spectra_product(gload_in<LP>(src, gi),
+							gload_in<LP>(src2, gi), c);
 				}
 			} else if (FIRST_ROUND && RD::DIRECT && LIFT && c.lift_add) {
 				const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
 				for (int e = 0; e < 8; e++)
-					v[e] = lift_word(gload_in<G::P>(src, g0 +
+					v[e] = lift_word(gload_in<LP>(src, g0 +
 							tile_off<G>((uint32_t) e << F)), c);
 			} else if (FIRST_ROUND && RD::DIRECT) {
 				const uint32_t g0 = tile_addr<G>(l0, jbase);
 #pragma unroll
 				for (int e = 0; e < 8; e++)
-					v[e] = gload_in<G::P>(src, g0 + tile_off<G>((uint32_t) e << F));
+					v[e] = gload_in<LP>(src, g0 + tile_off<G>((uint32_t) e << F));
 			} else {
 				const uint32_t p0 = pad(l0);
 #pragma unroll
@@ -1147,13 +1151,16 @@ __attribute__((amdgpu_waves_per_eu(pm_waves(LOGN, TB)))) void k_ntt_pm(
 		const uint32_t jbase = PASS == 0 ? 0 : blockIdx.x << TB;
 		const auto seq = std::make_integer_sequence<int, G::ROUNDS>();
 		// b's forward rounds, then a's, both kept in registers in the same
-		// layout (thread t holds words 8t..8t+7 after the last round)
+		// layout (thread t holds words 8t..8t+7 after the last round).  Both
+		// operands are read once, with the nontemporal hint, so that they
+		// leave the L2 to the two twiddle tables the kernel rereads (2^15 to
+		// 2^17 measured 1.5-3% faster than with ordinary loads).
 		uint64_t vb[8], v[8];
-		run_rounds_v<G, false, LOGN, false, MC, kIoKeep>(seq, vb, t, k.in2,
+		run_rounds_v<G, false, LOGN, false, MC, kIoKeep | kIoStreamIn>(seq, vb, t, k.in2,
 				nullptr, k.out, lds, jbase, jbase, k.tw2, k.c, true);
 		// (a's first round may write LDS words other waves read in b's last)
 		__syncthreads();
-		run_rounds_v<G, false, LOGN, false, MC, kIoKeep>(seq, v, t, k.in,
+		run_rounds_v<G, false, LOGN, false, MC, kIoKeep | kIoStreamIn>(seq, v, t, k.in,
 				nullptr, k.out, lds, jbase, jbase, k.tw2, k.c, true);
 		if (G::THREADS >= 64 || t < (uint32_t) G::THREADS) {
 #pragma unroll
