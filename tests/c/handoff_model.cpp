@@ -116,7 +116,7 @@ struct Shared {
 	std::atomic<uint32_t> done[kXcds];
 	std::atomic<int> gave_up{0};
 	std::atomic<int> stale{0};
-	std::atomic<uint64_t> max_spins{0};
+	std::atomic<uint64_t> max_wait_us{0};
 	std::vector<std::atomic<uint32_t>> count;
 	std::vector<uint8_t> inter;   // producer tile markers (plain stores)
 	std::vector<uint8_t> out;     // consumer tile markers
@@ -132,7 +132,11 @@ struct Shared {
 	}
 };
 
-constexpr uint64_t kSpinLimit = 1ull << 22;
+// A wait gives up after this long.  The kernel bounds its polls by count;
+// host threads share few CPUs with many others, so the model bounds them
+// by time: long enough that a correct schedule never reaches it even on a
+// loaded machine, short enough for the planted deadlock to end the run.
+static std::chrono::milliseconds g_wait_limit(30000);
 
 // One resident slot of XCD x: runs workgroups one after another while the
 // XCD has workgroups left to launch; each claims one ticket.
@@ -153,17 +157,25 @@ static void worker(const Run &r, Shared &s, uint32_t x, uint32_t seed) {
 			s.inter[w.poly * r.tiles + w.tile] = 1;
 			s.count[w.poly].fetch_add(1, std::memory_order_release);
 		} else {
+			const auto t0 = std::chrono::steady_clock::now();
 			uint64_t spins = 0;
-			while (s.count[w.poly].load(std::memory_order_acquire) < r.tiles) {
-				if (++spins == kSpinLimit) {
-					s.gave_up++;
-					break;
-				}
-				if ((spins & 255) == 0)
+			// (as in the kernel, once any wait has given up none waits again)
+			while (!s.gave_up.load() &&
+					s.count[w.poly].load(std::memory_order_acquire) < r.tiles) {
+				if ((++spins & 255) == 0) {
+					// let the producers' threads run; now and then check the clock
 					std::this_thread::yield();
+					if ((spins & 16383) == 0 &&
+							std::chrono::steady_clock::now() - t0 > g_wait_limit) {
+						s.gave_up++;
+						break;
+					}
+				}
 			}
-			uint64_t m = s.max_spins.load();
-			while (spins > m && !s.max_spins.compare_exchange_weak(m, spins)) {
+			const uint64_t us = (uint64_t) std::chrono::duration_cast<
+				std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
+			uint64_t m = s.max_wait_us.load();
+			while (us > m && !s.max_wait_us.compare_exchange_weak(m, us)) {
 			}
 			for (uint32_t t = 0; t < r.tiles; t++)
 				if (!s.inter[w.poly * r.tiles + t])
@@ -191,15 +203,16 @@ static int simulate(const Run &r, uint32_t seed) {
 		for (uint32_t t = 0; t < r.tiles; t++)
 			all &= s.out[p * r.tiles + t] == 1;
 	printf("npoly=%u tiles=%u resident=%u%s: gave_up=%d stale=%d complete=%d "
-			"max_spins=%llu\n", r.npoly, r.tiles, r.resident,
+			"max_wait_us=%llu\n", r.npoly, r.tiles, r.resident,
 			r.planted ? " (planted)" : "", s.gave_up.load(), s.stale.load(),
-			(int) all, (unsigned long long) s.max_spins.load());
+			(int) all, (unsigned long long) s.max_wait_us.load());
 	return (s.gave_up || s.stale || !all) ? 1 : 0;
 }
 
 int main(int argc, char **argv) {
 	const bool planted = argc > 1 && !strcmp(argv[1], "--planted");
 	if (planted) {
+		g_wait_limit = std::chrono::milliseconds(1000);
 		// one resident workgroup per XCD: the moved consumer block waits for
 		// producers that only a later workgroup would run
 		const Run moved{64, 16, 1, true, 0, 1};
