@@ -15,6 +15,12 @@ typedef void (*FusedKernel)(NttTask, const NttTask *);
 
 typedef void (*PassKernel)(NttTask, const NttTask *, uint32_t);
 
+// Flags, the third argument of the pass kernels.  kSecondPass: the
+// second pass of a direction.  kReversed: workgroup row y takes polynomial
+// gridDim.y - 1 - y, so that a pass starts on the polynomials the previous
+// launch wrote last, which are the ones still held in the last-level cache.
+constexpr uint32_t kSecondPass = 1, kReversed = 2;
+
 // Both passes of a two-pass size in one persistent grid (k_ntt_handoff):
 // (uniform, tasks, polynomials, state words, spin limit).
 typedef void (*HandoffKernel)(NttTask, const NttTask *, uint32_t, uint32_t *,
@@ -950,8 +956,9 @@ __attribute__((amdgpu_waves_per_eu(fused_waves(LOGN, TB)))) void k_ntt_fused(
 }
 
 // One pass of a two-pass size, one workgroup per (tile, polynomial):
-// blockIdx.x = tile, blockIdx.y = polynomial.  The first pass of a
-// direction reads `in`, the second reads and writes `out`.
+// blockIdx.x = tile, blockIdx.y = polynomial (counted from the last with
+// kReversed).  The first pass of a direction reads `in`, the second
+// (kSecondPass) reads and writes `out`.
 // Waves per SIMD the pass kernels are compiled for: eight (at most 64
 // VGPRs), which every shape fits without spilling except pass 1 of 2^20 on
 // 2^13-word tiles (ten stages over eight columns), which needs about 96.
@@ -964,15 +971,18 @@ constexpr int pass_waves(int logn, int pass, int tb) {
 template <int LOGN, int PASS, bool INV, int TB, int MC, int SPLIT = 0>
 __global__ __launch_bounds__((FusedShape<LOGN, TB>::THREADS))
 __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass(
-		NttTask uniform, const NttTask *tasks, uint32_t second) {
+		NttTask uniform, const NttTask *tasks, uint32_t flags) {
 	__shared__ uint64_t lds[FusedShape<LOGN, TB>::LDS_WORDS];
 	if constexpr (LOGN > TB) {
+		const bool second = flags & kSecondPass;
+		const uint32_t y = flags & kReversed ? gridDim.y - 1 - blockIdx.y
+				: blockIdx.y;
 		NttTask k;
 		if (tasks) {
-			k = tasks[blockIdx.y];
+			k = tasks[y];
 		} else {
 			k = uniform;
-			const uint64_t off = (uint64_t) blockIdx.y << LOGN;
+			const uint64_t off = (uint64_t) y << LOGN;
 			k.in += off;
 			k.out += off;
 			if (k.in2)
@@ -984,11 +994,11 @@ __attribute__((amdgpu_waves_per_eu(pass_waves(LOGN, PASS, TB)))) void k_ntt_pass
 		const uint32_t gbase = PASS == 2 ? jbase : 0;
 		// in2: the product operand of an inverse's first pass, the
 		// epilogue operand of a forward's second
-		const bool with_in2 = INV ? second == 0 : second != 0;
+		const bool with_in2 = INV ? !second : second;
 		run_rounds<G, INV, LOGN, false, MC>(
 				std::make_integer_sequence<int, G::ROUNDS>(), threadIdx.x,
 				second ? k.out : k.in, with_in2 ? k.in2 : nullptr, k.out, lds, jbase,
-				gbase, k.tw, k.c, second != 0);
+				gbase, k.tw, k.c, second);
 	}
 }
 
