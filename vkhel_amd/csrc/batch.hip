@@ -269,13 +269,35 @@ uint64_t job_transform_coeffs(const Job &j) {
 	}
 }
 
-// Sorts tasks by log2 n and modulus class and records the runs.
-static std::vector<Segment> segment(std::vector<NttTask> &tasks) {
+// Sorts tasks by log2 n and modulus class and records the runs.  With
+// by_table the tasks of a run of n >= 2^kGroupByTableLogn are also grouped
+// by their twiddle table (stable: a table's tasks keep their order), so
+// that the workgroups in flight share one table in each XCD's 4 MB L2: a
+// row pass reads about 2^logn table entries of 16 bytes per modulus
+// (2 MiB at n = 2^17), and eight moduli interleaved task by task would
+// want 16 MiB.  Measured on MI355X, 8 moduli interleaved, forward ms:
+// 256 x 2^17 0.228 -> 0.221, 128 x 2^18 0.252 -> 0.244, 64 x 2^19
+// 0.272 -> 0.264, 32 x 2^20 0.340 -> 0.311.  Smaller sizes
+// lose by it (512 x 2^16 0.199 -> 0.214: the polynomials in flight then
+// lie far apart in memory instead of side by side) and keep their order;
+// so do polynomial products, whose column and row tasks are matched by
+// index.
+constexpr uint32_t kGroupByTableLogn = 17;
+
+static std::vector<Segment> segment(std::vector<NttTask> &tasks,
+		bool by_table) {
 	const auto key = [](const NttTask &k) {
 		return 3 * k.c.logn + (uint32_t) modulus_class(k.c.q);
 	};
 	std::stable_sort(tasks.begin(), tasks.end(),
-			[&](const NttTask &a, const NttTask &b) { return key(a) < key(b); });
+			[&](const NttTask &a, const NttTask &b) {
+				const uint32_t ka = key(a), kb = key(b);
+				if (ka != kb)
+					return ka < kb;
+				// same key, so the same n
+				return by_table && a.c.logn >= kGroupByTableLogn &&
+					(uintptr_t) a.tw < (uintptr_t) b.tw;
+			});
 	std::vector<Segment> segs;
 	for (uint32_t i = 0; i < tasks.size(); i++) {
 		const int mc = modulus_class(tasks[i].c.q);
@@ -502,11 +524,11 @@ int plan_tasks(vkhel_ctx *ctx, const Job *const *jobs, size_t njobs,
 	order_pointwise(ts.pt);
 	order_pointwise(ts.post);
 	order_pointwise(ts.post2);
-	ts.seg_fwd = segment(ts.fwd);
-	ts.seg_inv = segment(ts.inv);
-	ts.seg_fwd2 = segment(ts.fwd2);
-	ts.seg_pm_col = segment(ts.pm_col);
-	ts.seg_pm = segment(ts.pm);
+	ts.seg_fwd = segment(ts.fwd, true);
+	ts.seg_inv = segment(ts.inv, true);
+	ts.seg_fwd2 = segment(ts.fwd2, true);
+	ts.seg_pm_col = segment(ts.pm_col, false);
+	ts.seg_pm = segment(ts.pm, false);
 	for (auto *segs : {&ts.seg_pm_col, &ts.seg_pm}) {
 		for (Segment &g : *segs)
 			g.tb = polymul_fused_tile(ctx, g.logn);
