@@ -1292,7 +1292,9 @@ def test_batch_with_70000_limbs(ctx):
     L = 70000
     q = O.primes_ntt(1 << 50, 1 << 13, 1)[0]
     rng = np.random.default_rng(70000)
-    sample = rng.choice(L, 24, replace=False).tolist() + [0, 65534, 65535, L - 1]
+    # 0, the grid's y limit, and chunk edges (2^25 words: 8192 limbs at 2^12)
+    sample = rng.choice(L, 24, replace=False).tolist() + [
+        0, 8191, 8192, 16383, 16384, 65534, 65535, L - 1]
     for n in (8, 1 << 12):
         t = V.NttTables(n, q, O.root_2n(q, n))
         x = rng.integers(0, q, L * n, dtype=np.uint64)
@@ -1319,6 +1321,48 @@ def test_batch_with_70000_limbs(ctx):
             ctx.set_tile_bits(12, 0)
         a.close()
         r.close()
+        t.close()
+
+
+@pytest.mark.parametrize("logn,L,tb", [(16, 513, 11), (16, 513, 12),
+                                       (17, 257, 12)])
+def test_two_pass_batch_across_chunk_edge(ctx, logn, L, tb):
+    """One chunk of 2^25 words and one of a single polynomial: L limbs
+    over three moduli interleaved limb by limb, in one batch job.  The second
+    pass walks each chunk's polynomials from the last, and from 2^17 up the
+    tasks are grouped by table first; every limb's forward must equal the
+    oracle's, and the inverse batch must give the input back."""
+    n = 1 << logn
+    qs = O.primes_ntt(1 << 50, 2 * n, 3)
+    ws = [O.root_2n(q, n) for q in qs]
+    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+    per = [tabs[i % 3] for i in range(L)]
+    rng = np.random.default_rng(L + tb)
+    x = np.concatenate([rng.integers(0, qs[i % 3], n, dtype=np.uint64)
+                        for i in range(L)])
+    a, r, back = vec(ctx, x), ctx.vector(L * n), ctx.vector(L * n)
+    ctx.set_tile_bits(logn, tb)
+    ctx.set_pass_mode(logn, 2)
+    try:
+        fwd, inv = V.Batch(ctx), V.Batch(ctx)
+        fwd.add(V.JOB_NTT_FWD, a, None, r, per)
+        inv.add(V.JOB_NTT_INV, r, None, back, per)
+        fwd.run()
+        got = r.read()
+        for i in range(L):
+            sl = slice(i * n, (i + 1) * n)
+            assert np.array_equal(got[sl], O.ntt_forward(x[sl], qs[i % 3],
+                                                         ws[i % 3])), i
+        inv.run()
+        assert np.array_equal(back.read(), x)
+        fwd.close()
+        inv.close()
+    finally:
+        ctx.set_tile_bits(logn, 0)
+        ctx.set_pass_mode(logn, 0)
+    for v in (a, r, back):
+        v.close()
+    for t in tabs:
         t.close()
 
 
