@@ -192,14 +192,14 @@ int tile_bits_for(const vkhel_ctx *ctx, uint32_t logn) {
 // pass walks the chunk's polynomials from the last (kReversed): it starts
 // on what the first pass wrote last, still held in the 256 MB last-level
 // cache, and the next launch's first pass, walking forward, starts where
-// this one ended.  Measured on MI355X (forward, 2^16 words, 2^11 tiles, one
-// box, interleaved runs): 512 polynomials 0.212 ms in two 2^24-word chunks
-// in order, 0.214 with the second pass reversed, 0.200 in one reversed
-// chunk; 2048 polynomials 0.920 -> 0.876 ms (2^25-word reversed chunks
-// against 2^24 in order); 256 x 2^17 0.238 -> 0.224, 32 x 2^20 0.348 ->
-// 0.332, 1024 x 2^15 0.199 -> 0.189.  Smaller chunks add launch tails
-// (2^23-word reversed chunks: 0.236 at 512 x 2^16).  Chunks also keep the
-// grid's y dimension within its limit of 65535.
+// this one ended.  Measured on MI355X (forward, 2^16 words, 2^11 tiles,
+// variants run back to back on one box): 512 polynomials 0.212 ms in two
+// 2^24-word chunks in order, 0.214 with the second pass reversed, 0.200 in
+// one reversed chunk; 2048 polynomials 0.920 -> 0.876 ms (2^25-word
+// reversed chunks against 2^24 in order); 256 x 2^17 0.238 -> 0.224,
+// 32 x 2^20 0.348 -> 0.332, 1024 x 2^15 0.199 -> 0.189.  Smaller chunks add
+// launch tails (2^23-word reversed chunks: 0.236 at 512 x 2^16).  Chunks
+// also keep the grid's y dimension within its limit of 65535.
 constexpr uint64_t kChunkWords = 1ull << 25;
 
 // The kernels of a two-pass size on a context: pass 1 (columns) from the
