@@ -1133,7 +1133,8 @@ __device__ __forceinline__ u64x2 gload2(const uint64_t *p, uint32_t i) {
 // (16 VGPRs more than a pass kernel) and runs at five waves per SIMD, where
 // it needs no scratch (87 VGPRs).  At six it spilled 52 bytes per thread
 // and measured 5-6% slower on 2^27-word batches of products at n = 2^15 to
-// 2^17 on MI355X.
+// 2^17 on MI355X; at four (on 2^12 tiles the same two 512-thread
+// workgroups per CU, with 128 VGPRs) 1-3% slower at 2^15 and 2^16.
 constexpr int pm_waves(int logn, int tb) {
 	return logn <= tb ? 1 : 5;
 }
