@@ -552,6 +552,64 @@ def test_batch_roundtrip_full_size(ctx):
         assert np.array_equal(v.read(), x)
 
 
+def _eval_at(a, z, q):
+    # a(z) mod q by Horner's rule in exact integers
+    acc = 0
+    for x in reversed(a.tolist()):
+        acc = (acc * z + x) % q
+    return acc
+
+
+def test_headline_batch_properties(ctx):
+    """The headline workload at its full size: 512 polynomials of n = 2^16
+    over 8 primes < 2^50, interleaved polynomial by polynomial, one batch
+    per direction.  Properties that hold at any size: the forward transform
+    is linear (NTT(a) + NTT(b) = NTT(a + b) mod q on every polynomial);
+    sampled outputs equal the polynomial evaluated at psi^(2 bitrev(i) + 1),
+    in exact integers and independent of the oracle; and the inverse gives
+    every input back."""
+    n, primes, per = 1 << 16, 8, 64
+    L = primes * per
+    qs = O.primes_ntt(1 << 50, 2 * n, primes)
+    ws = [O.root_2n(q, n) for q in qs]
+    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+    qcol = np.array([qs[i % primes] for i in range(L)], dtype=np.uint64)[:, None]
+    rng = np.random.default_rng(1 << 16)
+    a = rng.integers(0, 1 << 62, (L, n), dtype=np.uint64) % qcol
+    b = rng.integers(0, 1 << 62, (L, n), dtype=np.uint64) % qcol
+    c = (a + b) % qcol
+    slabs = [vec(ctx, m.ravel()) for m in (a, b, c)]
+    views = [[s.view(i * n, n) for i in range(L)] for s in slabs]
+    fwd, inv = V.Batch(ctx), V.Batch(ctx)
+    for vs in views:
+        for i, v in enumerate(vs):
+            fwd.add(V.JOB_NTT_FWD, v, None, v, [tabs[i % primes]])
+    for i, v in enumerate(views[0]):
+        inv.add(V.JOB_NTT_INV, v, None, v, [tabs[i % primes]])
+    fwd.run()
+    fa, fb, fc = (s.read().reshape(L, n) for s in slabs)
+    assert np.array_equal((fa + fb) % qcol, fc)
+    rng2 = np.random.default_rng(7)
+    sample = [(0, 0), (1, 1), (L - 1, n - 1), (L // 2, 12345)] + [
+        (int(p), int(i)) for p, i in zip(rng2.integers(0, L, 6),
+                                         rng2.integers(0, n, 6))]
+    for p, i in sample:
+        q, w = qs[p % primes], ws[p % primes]
+        z = pow(w, 2 * R.bitrev(i, 16) + 1, q)
+        assert int(fa[p, i]) == _eval_at(a[p], z, q), (p, i)
+    inv.run()
+    assert np.array_equal(slabs[0].read().reshape(L, n), a)
+    fwd.close()
+    inv.close()
+    for vs in views:
+        for v in vs:
+            v.close()
+    for s in slabs:
+        s.close()
+    for t in tabs:
+        t.close()
+
+
 # ---------------- tile sizes, split vs whole-polynomial kernels ----------------
 
 @pytest.mark.parametrize("bits", [55, 59])
