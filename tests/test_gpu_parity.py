@@ -610,6 +610,52 @@ def test_headline_batch_properties(ctx):
         t.close()
 
 
+def test_polymul_batch_properties(ctx):
+    """The polynomial-product workload at its full size: 256 products of
+    16-limb RNS polynomials (16 primes < 2^55, n = 2^15) in one batch.  For
+    the negacyclic product c = a b mod (X^n + 1, q), c(z) = a(z) b(z) mod q
+    at every root z of X^n + 1, i.e. at the odd powers of psi: sampled limbs
+    are checked there by Horner's rule in exact integers, independent of
+    any transform, and one product's every limb against the oracle."""
+    n, primes, pairs = 1 << 15, 16, 256
+    qs = O.primes_ntt(1 << 55, 2 * n, primes)
+    ws = [O.root_2n(q, n) for q in qs]
+    tabs = [V.NttTables(n, q, w) for q, w in zip(qs, ws)]
+    qcol = np.array(qs, dtype=np.uint64)[:, None]
+    rng = np.random.default_rng(1 << 15)
+    xs, vs = [], []
+    batch = V.Batch(ctx)
+    for j in range(pairs):
+        xa = rng.integers(0, 1 << 62, (primes, n), dtype=np.uint64) % qcol
+        xb = rng.integers(0, 1 << 62, (primes, n), dtype=np.uint64) % qcol
+        a, b, r = vec(ctx, xa.ravel()), vec(ctx, xb.ravel()), ctx.vector(primes * n)
+        batch.add(V.JOB_POLYMUL, a, b, r, tabs)
+        xs.append((xa, xb))
+        vs.append((a, b, r))
+    batch.run()
+    rng2 = np.random.default_rng(15)
+    for j, l, k in zip(rng2.integers(0, pairs, 12), rng2.integers(0, primes, 12),
+                       rng2.integers(0, n, 12)):
+        j, l = int(j), int(l)
+        q = qs[l]
+        z = pow(ws[l], 2 * int(k) + 1, q)
+        xa, xb = xs[j]
+        c = vs[j][2].read()[l * n:(l + 1) * n]
+        assert _eval_at(c, z, q) == _eval_at(xa[l], z, q) * _eval_at(xb[l], z, q) % q, \
+            (j, l, int(k))
+    xa, xb = xs[pairs - 1]
+    got = vs[pairs - 1][2].read()
+    for l, q in enumerate(qs):
+        want = O.polymul_ntt(xa[l], xb[l], q, ws[l])
+        assert np.array_equal(got[l * n:(l + 1) * n], want), l
+    batch.close()
+    for trio in vs:
+        for v in trio:
+            v.close()
+    for t in tabs:
+        t.close()
+
+
 # ---------------- tile sizes, split vs whole-polynomial kernels ----------------
 
 @pytest.mark.parametrize("bits", [55, 59])
